@@ -1,0 +1,64 @@
+// Shared definitions for the gfx950 kernels of the VMC hot path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16;
+
+#define JQ_WAVE 64
+
+// Tangent-row convention used by every Laplacian-mode buffer: for each unit
+// (an electron's feature vector, an orbital-matrix element, ...) the rows are
+//   row 0        value
+//   row 1        Laplacian over all 3N electron coordinates
+//   row 2 + c    d/dx_c, c = 3*electron + axis
+// padded with zero rows up to a multiple of 16 ("Rp").
+#define ROW_VAL 0
+#define ROW_LAP 1
+#define ROW_J0 2
+
+// Unit (spin-blocked electron) ordering of all per-electron buffers in a
+// walker chunk of Bc walkers: all spin-up electrons of all walkers first
+// (walker-major), then all spin-down electrons.  A GEMM tile or a spin-channel
+// weight choice therefore never straddles channels.
+__host__ __device__ inline int unit_of(int w, int i, int Bc, int n_up, int n_dn) {
+  return i < n_up ? w * n_up + i : Bc * n_up + w * n_dn + (i - n_up);
+}
+__host__ __device__ inline int walker_of_unit(int u, int Bc, int n_up, int n_dn) {
+  const int uu = Bc * n_up;
+  return u < uu ? u / n_up : (u - uu) / n_dn;
+}
+__host__ __device__ inline int elec_of_unit(int u, int Bc, int n_up, int n_dn) {
+  const int uu = Bc * n_up;
+  return u < uu ? u % n_up : n_up + (u - uu) % n_dn;
+}
+
+template <typename T> __device__ inline float to_f(T x);
+template <> __device__ inline float to_f<float>(float x) { return x; }
+template <> __device__ inline float to_f<bf16>(bf16 x) { return (float)x; }
+template <typename T> __device__ inline T from_f(float x);
+template <> __device__ inline float from_f<float>(float x) { return x; }
+template <> __device__ inline bf16 from_f<bf16>(float x) { return (bf16)x; }
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double wave_max_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+static inline int ceil_div(long x, long m) { return (int)((x + m - 1) / m); }

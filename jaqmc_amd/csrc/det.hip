@@ -1,0 +1,428 @@
+// FP64 determinants of the orbital matrices and their forward Laplacian.
+//
+// One workgroup per (walker, determinant).  The orbital matrix
+// A[i][o] = phi[i][o] * env[i][o] (electron i, orbital o) is formed in FP64 in
+// LDS from the FP32 orbital-projection output and the envelope (product rule,
+// laplacian/primitives/core.py:494-500), inverted by Gauss-Jordan elimination
+// with partial pivoting, and log|det A| / sign taken from the pivots (the
+// values of jnp.linalg.slogdet, wavefunction/output/logdet.py:65).
+//
+// Laplacian mode restates laplacian/primitives/slogdet.py:44-59:
+//   d logdet / dx_c = tr(A^-1 J_c)
+//   lap logdet      = tr(A^-1 L) - sum_c tr((A^-1 J_c)^2)
+// with B_c = A^-1 J_c computed in 4x4 register tiles.  A thread owns the tile
+// pair (P, Q) and (Q, P) of B_c, so tr(B_c^2) = sum_{P,Q} <B_PQ, B_QP^T> needs no
+// second pass; the diagonal tiles also give tr(B_c).
+
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace {
+
+constexpr int DET_THREADS = 256;
+
+// Orbital-matrix element (value, and optionally Jacobian row c or Laplacian).
+__device__ inline const float* env_ptr(const FermiDims& d, const float* env, int w, int i, int k, int o) {
+  return env + ((((long)w * d.N + i) * d.K + k) * d.N + o) * 5;
+}
+
+__global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+                                                          const float* __restrict__ env, double* __restrict__ out,
+                                                          int Np, int cpi, int lapmode) {
+  extern __shared__ double sm[];
+  const int N = d.N, T = 3 * N;
+  const int w = blockIdx.x / d.K, k = blockIdx.x % d.K;
+  const int tid = threadIdx.x;
+  double* A = sm;                // Np*Np   (later: inverse transposed)
+  double* Inv = sm + Np * Np;    // Np*Np   (later: J_c buffers)
+  double* red = sm + Np * Np + (long)max(Np * Np, cpi * Np * Np);  // scratch: 64 doubles + grad[T]
+  int* ipiv = (int*)(red + 64 + T);
+  const int Rp = d.Rp;
+
+  // ---- load A = phi * env, pad with identity ----
+  for (int idx = tid; idx < Np * Np; idx += DET_THREADS) {
+    const int i = idx / Np, o = idx % Np;
+    double a;
+    if (i < N && o < N) {
+      const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+      const double phi = Phi[(long)u * Rp * ldphi + k * N + o];
+      a = phi * (double)env_ptr(d, env, w, i, k, o)[0];
+    } else {
+      a = (i == o) ? 1.0 : 0.0;
+    }
+    A[idx] = a;
+    Inv[idx] = (i == o) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+
+  // ---- Gauss-Jordan with partial pivoting on [A | I] ----
+  double logdet = 0.0;
+  int sgn = 1;
+  for (int p = 0; p < Np; ++p) {
+    if (tid < 64) {
+      // argmax |A[i][p]|, i >= p (one wave)
+      double best = -1.0;
+      int bi = p;
+      for (int i = p + tid; i < Np; i += 64) {
+        const double v = fabs(A[i * Np + p]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) ipiv[0] = bi;
+    }
+    __syncthreads();
+    const int q = ipiv[0];
+    if (q != p) {
+      for (int c = tid; c < Np; c += DET_THREADS) {
+        double t = A[p * Np + c];
+        A[p * Np + c] = A[q * Np + c];
+        A[q * Np + c] = t;
+        t = Inv[p * Np + c];
+        Inv[p * Np + c] = Inv[q * Np + c];
+        Inv[q * Np + c] = t;
+      }
+      sgn = -sgn;
+    }
+    __syncthreads();
+    const double pv = A[p * Np + p];
+    logdet += log(fabs(pv));
+    if (pv < 0) sgn = -sgn;
+    const double rp = 1.0 / pv;
+    // factors of column p (read before the row update)
+    for (int i = tid; i < Np; i += DET_THREADS) red[i] = (i == p) ? 0.0 : A[i * Np + p];
+    __syncthreads();
+    for (int c = tid; c < Np; c += DET_THREADS) {
+      A[p * Np + c] *= rp;
+      Inv[p * Np + c] *= rp;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < Np * 2 * Np; idx += DET_THREADS) {
+      const int i = idx / (2 * Np), c = idx % (2 * Np);
+      if (i == p) continue;
+      const double f = red[i];
+      if (c < Np)
+        A[i * Np + c] -= f * A[p * Np + c];
+      else
+        Inv[i * Np + (c - Np)] -= f * Inv[p * Np + (c - Np)];
+    }
+    __syncthreads();
+  }
+
+  double* o_ = out + (long)blockIdx.x * (3 + (lapmode ? T : 0));
+  if (!lapmode) {
+    if (tid == 0) {
+      o_[0] = logdet;
+      o_[1] = (double)sgn;
+    }
+    return;
+  }
+
+  // ---- InvT[i][o] = Inv[o][i] into A's space ----
+  for (int idx = tid; idx < Np * Np; idx += DET_THREADS) {
+    const int i = idx / Np, o = idx % Np;
+    A[idx] = Inv[o * Np + i];
+  }
+  for (int c = tid; c < T; c += DET_THREADS) red[64 + c] = 0.0;
+  __syncthreads();
+  double* InvT = A;
+  double* Jb = Inv;
+
+  // ---- L[i][o] (product rule with the envelope) and tr(A^-1 L) ----
+  double part = 0.0;
+  for (int idx = tid; idx < N * N; idx += DET_THREADS) {
+    const int i = idx / N, o = idx % N;
+    const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+    const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
+    const float* e = env_ptr(d, env, w, i, k, o);
+    double L = (double)ph[ROW_LAP * ldphi] * e[0] + (double)ph[0] * e[1];
+    for (int ax = 0; ax < 3; ++ax) L += 2.0 * (double)ph[(ROW_J0 + 3 * i + ax) * ldphi] * e[2 + ax];
+    part += InvT[i * Np + o] * L;
+  }
+  part = wave_sum_d(part);
+  if ((tid & 63) == 0) red[tid >> 6] = part;
+  __syncthreads();
+  const double trAL = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
+
+  // ---- sum_c tr((A^-1 J_c)^2) and tr(A^-1 J_c) ----
+  const int nb = Np / 4;
+  const int npairs = nb * (nb + 1) / 2;
+  const int my_cc = tid / npairs;
+  int P = 0, Q = 0;
+  {
+    int rem = tid % npairs;
+    // enumerate P <= Q row by row
+    while (rem >= nb - P) {
+      rem -= nb - P;
+      ++P;
+    }
+    Q = P + rem;
+  }
+  double tsum = 0.0;
+  for (int c0 = 0; c0 < T; c0 += cpi) {
+    const int ncc = min(cpi, T - c0);
+    for (int idx = tid; idx < ncc * Np * Np; idx += DET_THREADS) {
+      const int cc = idx / (Np * Np);
+      const int io = idx % (Np * Np);
+      const int i = io / Np, o = io % Np;
+      double j = 0.0;
+      if (i < N && o < N) {
+        const int c = c0 + cc;
+        const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+        const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
+        const float* e = env_ptr(d, env, w, i, k, o);
+        j = (double)ph[(ROW_J0 + c) * ldphi] * e[0];
+        if (c / 3 == i) j += (double)ph[0] * e[2 + c % 3];
+      }
+      Jb[idx] = j;
+    }
+    __syncthreads();
+    if (my_cc < ncc) {
+      const double* J = Jb + (long)my_cc * Np * Np;
+      double b1[4][4], b2[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) b1[a][b] = b2[a][b] = 0.0;
+      for (int i = 0; i < Np; ++i) {
+        const double* ir = InvT + i * Np;
+        const double* jr = J + i * Np;
+        double ip[4], jq[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          ip[a] = ir[4 * P + a];
+          jq[a] = jr[4 * Q + a];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) b1[a][b] += ip[a] * jq[b];
+        if (P != Q) {
+          double iq[4], jp[4];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            iq[a] = ir[4 * Q + a];
+            jp[a] = jr[4 * P + a];
+          }
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) b2[a][b] += iq[a] * jp[b];
+        }
+      }
+      double t = 0.0;
+      if (P == Q) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) t += b1[a][b] * b1[b][a];
+        const double tr = b1[0][0] + b1[1][1] + b1[2][2] + b1[3][3];
+        atomicAdd(&red[64 + c0 + my_cc], tr);
+      } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) t += b1[a][b] * b2[b][a];
+        t *= 2.0;
+      }
+      tsum += t;
+    }
+    __syncthreads();
+  }
+  tsum = wave_sum_d(tsum);
+  if ((tid & 63) == 0) red[tid >> 6] = tsum;
+  __syncthreads();
+  if (tid == 0) {
+    o_[0] = logdet;
+    o_[1] = (double)sgn;
+    o_[2] = trAL - (red[0] + red[1] + red[2] + red[3]);
+  }
+  for (int c = tid; c < T; c += DET_THREADS) o_[3 + c] = red[64 + c];
+}
+
+// ---------------------------------------------------------------------------
+// Per-walker combination: signed log-sum-exp over determinants
+// (wavefunction/output/logdet.py:66-79), its gradient and Laplacian, the
+// Psiformer Jastrow (wavefunction/jastrow.py:60-122), the Coulomb potential
+// (app/molecule/hamiltonian.py:9-22) and E_L = -1/2 (lap + |grad|^2) + V
+// (estimator/kinetic/_common.py:61-73).  One wavefront per walker, FP64.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void combine_kernel(FermiDims d, CombineArgs a) {
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int N = d.N, K = d.K, T = 3 * N;
+  const int stride = 3 + (a.lapmode ? T : 0);
+  const double* det = a.det + (long)w * K * stride;
+  const long gw = (long)a.B_off + w;
+  const float* x = a.elec + gw * N * 3;
+
+  double m = -INFINITY;
+  for (int k = 0; k < K; ++k) m = fmax(m, det[k * stride]);
+  double S = 0.0;
+  for (int k = 0; k < K; ++k) S += det[k * stride + 1] * exp(det[k * stride] - m);
+  const double sign = S > 0 ? 1.0 : (S < 0 ? -1.0 : 0.0);
+  double logpsi = log(fabs(S)) + m;
+
+  // Jastrow: J = sum_{i<j same} -1/4 a^2/(a+r) + sum_{up,dn} -1/2 a^2/(a+r)
+  double jv = 0.0, jl = 0.0;
+  if (a.jastrow) {
+    for (int p = lane; p < N * N; p += 64) {
+      const int i = p / N, j = p % N;
+      if (j <= i) continue;
+      const bool same = (i < d.n_up) == (j < d.n_up);
+      const double c = same ? 0.25 : 0.5;
+      const double al = same ? a.alpha_par : a.alpha_anti;
+      const double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1], dz = x[3 * i + 2] - x[3 * j + 2];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      const double q = al + r;
+      jv += -c * al * al / q;
+      const double f1 = c * al * al / (q * q), f2 = -2.0 * c * al * al / (q * q * q);
+      jl += 2.0 * (f2 + 2.0 * f1 / r);
+    }
+    jv = wave_sum_d(jv);
+    jl = wave_sum_d(jl);
+    logpsi += jv;
+  }
+
+  // potential
+  double v = 0.0;
+  for (int p = lane; p < N * d.M; p += 64) {
+    const int i = p / d.M, I = p % d.M;
+    const double dx = x[3 * i] - a.atoms[3 * I], dy = x[3 * i + 1] - a.atoms[3 * I + 1],
+                 dz = x[3 * i + 2] - a.atoms[3 * I + 2];
+    v -= a.charges[I] / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  for (int p = lane; p < N * N; p += 64) {
+    const int i = p / N, j = p % N;
+    if (j <= i) continue;
+    const double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1], dz = x[3 * i + 2] - x[3 * j + 2];
+    v += 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  for (int p = lane; p < d.M * d.M; p += 64) {
+    const int I = p / d.M, J = p % d.M;
+    if (J <= I) continue;
+    const double dx = a.atoms[3 * I] - a.atoms[3 * J], dy = a.atoms[3 * I + 1] - a.atoms[3 * J + 1],
+                 dz = a.atoms[3 * I + 2] - a.atoms[3 * J + 2];
+    v += (double)a.charges[I] * a.charges[J] / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  v = wave_sum_d(v);
+
+  if (!a.lapmode) {
+    if (lane == 0) {
+      a.logpsi[gw] = (float)logpsi;
+      if (a.sign) a.sign[gw] = (float)sign;
+    }
+    return;
+  }
+  // weights w_k = s_k exp(l_k - m) / S  (sum_k w_k = 1)
+  // grad = sum_k w_k g_k ;  lap = sum_k w_k (lap_k + |g_k|^2) - |grad|^2
+  double g2 = 0.0, wl = 0.0;
+  for (int c = lane; c < T; c += 64) {
+    double g = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double wk = det[k * stride + 1] * exp(det[k * stride] - m) / S;
+      const double gk = det[k * stride + 3 + c];
+      g += wk * gk;
+      wl += wk * gk * gk;
+    }
+    if (a.jastrow) {
+      const int i = c / 3, ax = c % 3;
+      for (int j = 0; j < N; ++j) {
+        if (j == i) continue;
+        const bool same = (i < d.n_up) == (j < d.n_up);
+        const double cc = same ? 0.25 : 0.5;
+        const double al = same ? a.alpha_par : a.alpha_anti;
+        const double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1], dz = x[3 * i + 2] - x[3 * j + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double q = al + r;
+        const double dax = ax == 0 ? dx : (ax == 1 ? dy : dz);
+        g += cc * al * al / (q * q) * dax / r;
+      }
+    }
+    if (a.grad) a.grad[gw * T + c] = (float)g;
+    g2 += g * g;
+  }
+  g2 = wave_sum_d(g2);
+  wl = wave_sum_d(wl);
+  double lapdet = 0.0;
+  for (int k = 0; k < K; ++k) lapdet += det[k * stride + 1] * exp(det[k * stride] - m) / S * det[k * stride + 2];
+  // lap log psi = sum_k w_k (lap_k + |g_k|^2) - |grad_det|^2 + lap J;  with the Jastrow
+  // the total gradient g already includes grad J, so recompute |grad_det|^2 implicitly:
+  // E_kin = -1/2 (lap + |g|^2) where lap = sum_k w_k(lap_k+|g_k|^2) - |g_det|^2 + lapJ and
+  // |g|^2 = |g_det + g_J|^2.  Evaluate with the explicit pieces.
+  double gdet2 = 0.0, gcross = 0.0;
+  if (a.jastrow) {
+    for (int c = lane; c < T; c += 64) {
+      double gd = 0.0;
+      for (int k = 0; k < K; ++k)
+        gd += det[k * stride + 1] * exp(det[k * stride] - m) / S * det[k * stride + 3 + c];
+      gdet2 += gd * gd;
+    }
+    gdet2 = wave_sum_d(gdet2);
+  } else {
+    gdet2 = g2;
+  }
+  (void)gcross;
+  const double lap = lapdet + wl - gdet2 + jl;
+  const double ekin = -0.5 * (lap + g2);
+  if (lane == 0) {
+    a.logpsi[gw] = (float)logpsi;
+    if (a.sign) a.sign[gw] = (float)sign;
+    a.e_kin[gw] = ekin;
+    a.e_pot[gw] = v;
+    a.e_loc[gw] = ekin + v;
+  }
+}
+
+}  // namespace
+
+size_t det_smem_bytes(int N, int* Np_out, int* cpi_out) {
+  const int Np = round_up(N, 4);
+  const int nb = Np / 4;
+  const int npairs = nb * (nb + 1) / 2;
+  int cpi = DET_THREADS / npairs;
+  if (cpi < 1) cpi = 1;
+  const long jb = 72 * 1024 / (8L * Np * Np);
+  if (cpi > jb) cpi = (int)(jb < 1 ? 1 : jb);
+  if (cpi > 3 * N) cpi = 3 * N;
+  *Np_out = Np;
+  *cpi_out = cpi;
+  const long T = 3L * N;
+  return (size_t)(8L * (Np * Np + (long)std::max(Np * Np, cpi * Np * Np) + 64 + T) + 16);
+}
+
+int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* env, double* out, hipStream_t s) {
+  int Np, cpi;
+  size_t smem = det_smem_bytes(d.N, &Np, &cpi);
+  if (d.Rp <= 1) {  // forward mode: no Jacobian buffers
+    cpi = 1;
+    smem = (size_t)(8L * (2L * Np * Np + 64 + 3L * d.N) + 16);
+  }
+  if (d.N > 64) return -1;
+  const int nb = Np / 4;
+  if (nb * (nb + 1) / 2 > DET_THREADS) return -1;
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipFuncSetAttribute((const void*)det_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(det_kernel, dim3(d.Bc * d.K), dim3(DET_THREADS), smem, s, d, Phi, ldphi, env, out, Np, cpi,
+                     d.Rp > 1 ? 1 : 0);
+  return 0;
+}
+
+int launch_combine(const FermiDims& d, const CombineArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(combine_kernel, dim3(d.Bc), dim3(64), 0, s, d, a);
+  return 0;
+}

@@ -1,0 +1,316 @@
+// FermiNet streaming kernels: input features, two-electron stream, spin-channel
+// aggregation and the envelope, all in the tangent-row layout of common.h.
+//
+// Forward-Laplacian bookkeeping follows the reference's rules
+// (laplacian/primitives/*.py) specialised to this graph:
+//   * the atom-electron features of electron i depend only on x_i (Local1):
+//     three non-zero Jacobian rows, written densely;
+//   * the pair stream h2[a][b] depends on x_a and x_b only (Local2): it is kept
+//     in 8 rows per pair -- value, Laplacian, d/dx_a (3), d/dx_b (3);
+//   * means over electrons are linear, so they are expanded to dense rows when
+//     aggregated (reductions.py:25-61).
+
+#include "kernels.h"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Layer-0 features (wavefunction/input/atomic.py:46-79, rescale=False):
+//   ae[i, 4I + 0] = |x_i - R_I|,  ae[i, 4I + 1 + d] = (x_i - R_I)_d
+//   ee[a, b, 0]  = |x_a - x_b| (0 on the diagonal), ee[a, b, 1 + d] = (x_a - x_b)_d
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void ae_features_kernel(FermiDims d, const float* __restrict__ elec, const float* __restrict__ atoms,
+                                   T* __restrict__ X, long ldx) {
+  const int F = 4 * d.M;
+  const long total = (long)d.Bc * d.N * d.Rp * F;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int f = idx % F;
+    const long ur = idx / F;
+    const int r = ur % d.Rp;
+    const int u = ur / d.Rp;
+    const int w = walker_of_unit(u, d.Bc, d.n_up, d.n_dn);
+    const int i = elec_of_unit(u, d.Bc, d.n_up, d.n_dn);
+    const int I = f >> 2, e = f & 3;
+    const float* xi = elec + ((long)w * d.N + i) * 3;
+    const float dx = xi[0] - atoms[3 * I], dy = xi[1] - atoms[3 * I + 1], dz = xi[2] - atoms[3 * I + 2];
+    const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float dv[3] = {dx, dy, dz};
+    float out = 0.f;
+    if (r == ROW_VAL) {
+      out = e == 0 ? rr : dv[e - 1];
+    } else if (r == ROW_LAP) {
+      out = e == 0 ? 2.f / rr : 0.f;
+    } else {
+      const int c = r - ROW_J0;
+      if (c < 3 * d.N && c / 3 == i) {
+        const int ax = c % 3;
+        out = e == 0 ? dv[ax] / rr : (ax == e - 1 ? 1.f : 0.f);
+      }
+    }
+    X[(long)u * d.Rp * ldx + (long)r * ldx + f] = from_f<T>(out);
+  }
+}
+
+// Pair-stream input, H2[w][a][b][row][4], RP2 = 8 rows (Laplacian) or 1.
+__global__ void ee_features_kernel(FermiDims d, int RP2, const float* __restrict__ elec, float* __restrict__ H2) {
+  const long total = (long)d.Bc * d.N * d.N;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < total; p += (long)gridDim.x * blockDim.x) {
+    const int b = p % d.N;
+    const int a = (p / d.N) % d.N;
+    const int w = p / ((long)d.N * d.N);
+    float* o = H2 + p * RP2 * 4;
+    const float* xa = elec + ((long)w * d.N + a) * 3;
+    const float* xb = elec + ((long)w * d.N + b) * 3;
+    const float dv[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
+    if (a == b) {
+      for (int q = 0; q < RP2 * 4; ++q) o[q] = 0.f;
+      continue;
+    }
+    const float rr = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    o[0] = rr;
+    o[1] = dv[0];
+    o[2] = dv[1];
+    o[3] = dv[2];
+    if (RP2 == 1) continue;
+    // Laplacian over both particles: 2 * (2/r) for r, 0 for displacements.
+    o[4] = 4.f / rr;
+    o[5] = o[6] = o[7] = 0.f;
+    for (int ax = 0; ax < 3; ++ax) {
+      float* ja = o + (2 + ax) * 4;
+      float* jb = o + (5 + ax) * 4;
+      ja[0] = dv[ax] / rr;
+      jb[0] = -dv[ax] / rr;
+      for (int e = 0; e < 3; ++e) {
+        ja[1 + e] = e == ax ? 1.f : 0.f;
+        jb[1 + e] = e == ax ? -1.f : 0.f;
+      }
+    }
+  }
+}
+
+// One two-electron layer: out = tanh(in W + b) (+ (in + out)/sqrt2 when
+// widths match), FermiLayers._update_double_stream (ferminet.py:57-63).
+// Thread = (pair, output feature); Local2 tanh rule over the 6 Jacobian rows.
+template <int RP2>
+__global__ void pair_layer_kernel(long npairs, const float* __restrict__ in, int fin, float* __restrict__ out,
+                                  int fout, const float* __restrict__ W, const float* __restrict__ bias,
+                                  int residual) {
+  const long total = npairs * fout;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int f = idx % fout;
+    const long p = idx / fout;
+    const float* x = in + p * RP2 * fin;
+    float acc[RP2];
+#pragma unroll
+    for (int r = 0; r < RP2; ++r) acc[r] = 0.f;
+    for (int k = 0; k < fin; ++k) {
+      const float wk = W[(long)k * fout + f];
+#pragma unroll
+      for (int r = 0; r < RP2; ++r) acc[r] += x[r * fin + k] * wk;
+    }
+    acc[0] += bias[f];
+    float* y = out + p * RP2 * fout;
+    const float t = tanhf(acc[0]);
+    const float d1 = 1.f - t * t;
+    float o[RP2];
+    o[0] = t;
+    if constexpr (RP2 > 1) {
+      float sq = 0.f;
+#pragma unroll
+      for (int r = 2; r < RP2; ++r) sq += acc[r] * acc[r];
+      o[1] = d1 * acc[1] - 2.f * t * d1 * sq;
+#pragma unroll
+      for (int r = 2; r < RP2; ++r) o[r] = d1 * acc[r];
+    }
+#pragma unroll
+    for (int r = 0; r < RP2; ++r) {
+      float v = o[r];
+      if (residual) v = (v + x[r * fin + f]) * 0.70710678118654752f;
+      y[r * fout + f] = v;
+    }
+  }
+}
+
+// g_two of FermiLayers.aggregate_features (ferminet.py:86-89): for electron i and
+// channel s, m_s[i] = mean_{j in s} h2[j][i].  Expanded to dense rows:
+//   d/dx_i   <- mean_j (d/dx_b rows of pair (j, i))
+//   d/dx_j   <- (d/dx_a rows of pair (j, i)) / n_s,   j in s
+// Columns [col0 + nch*f2, ldx) are zero-filled (GEMM K padding).
+template <typename T>
+__global__ void pair_means_kernel(FermiDims d, int RP2, const float* __restrict__ H2, int f2, T* __restrict__ X,
+                                  long ldx, int col0) {
+  const int ncol = ldx - col0;
+  const long total = (long)d.Bc * d.N * d.Rp * ncol;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int cc = idx % ncol;
+    const long ur = idx / ncol;
+    const int r = ur % d.Rp;
+    const int u = ur / d.Rp;
+    float out = 0.f;
+    if (cc < d.nch * f2) {
+      const int s = cc / f2, f = cc % f2;
+      const int w = walker_of_unit(u, d.Bc, d.n_up, d.n_dn);
+      const int i = elec_of_unit(u, d.Bc, d.n_up, d.n_dn);
+      const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
+      const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
+      const float inv = 1.f / ns;
+      // pair (j, i): H2[w][j][i]
+      const float* base = H2 + ((long)w * d.N * d.N + i) * RP2 * f2 + f;
+      const long jstride = (long)d.N * RP2 * f2;
+      if (r == ROW_VAL || r == ROW_LAP) {
+        float sacc = 0.f;
+        for (int j = j0; j < j0 + ns; ++j) sacc += base[j * jstride + r * f2];
+        out = sacc * inv;
+      } else if (r - ROW_J0 < 3 * d.N) {
+        const int c = r - ROW_J0;
+        const int e = c / 3, ax = c % 3;
+        if (e == i) {
+          float sacc = 0.f;
+          for (int j = j0; j < j0 + ns; ++j) sacc += base[j * jstride + (5 + ax) * f2];
+          out = sacc * inv;
+        } else if (e >= j0 && e < j0 + ns) {
+          out = base[e * jstride + (2 + ax) * f2] * inv;
+        }
+      }
+    }
+    X[(long)u * d.Rp * ldx + (long)r * ldx + col0 + cc] = from_f<T>(out);
+  }
+}
+
+// g_one: spin-channel means of the one-electron stream, per walker and row:
+// g[w][r][s*f1 + f] = mean_{i in s} X[unit(w, i)][r][f]; columns >= nch*f1 zeroed.
+template <typename T>
+__global__ void spin_means_kernel(FermiDims d, const T* __restrict__ X, long ldx, int f1, T* __restrict__ g,
+                                  long ldg) {
+  const long total = (long)d.Bc * d.Rp * ldg;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int cc = idx % ldg;
+    const long wr = idx / ldg;
+    const int r = wr % d.Rp;
+    const int w = wr / d.Rp;
+    float out = 0.f;
+    if (cc < d.nch * f1) {
+      const int s = cc / f1, f = cc % f1;
+      const int i0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
+      const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
+      float acc = 0.f;
+      for (int i = i0; i < i0 + ns; ++i) {
+        const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+        acc += to_f<T>(X[(long)u * d.Rp * ldx + (long)r * ldx + f]);
+      }
+      out = acc / ns;
+    }
+    g[idx] = from_f<T>(out);
+  }
+}
+
+// Envelope (wavefunction/output/envelope.py:111-140), per electron i of spin
+// channel s, det k, orbital o:  e = sum_I pi[o,I,k] exp(-a r_iI), a = |sigma| or
+// sigma.  Gradient wrt x_i: sum_I -a f (x_i - R_I)/r; Laplacian: sum_I f (a^2 - 2a/r).
+// Output env[w][i][k*N + o][5] = (value, Laplacian, d/dx, d/dy, d/dz).
+__global__ void envelope_kernel(FermiDims d, const float* __restrict__ elec, const float* __restrict__ atoms,
+                                const float* __restrict__ pi0, const float* __restrict__ sg0,
+                                const float* __restrict__ pi1, const float* __restrict__ sg1, int kind,
+                                float* __restrict__ env) {
+  const long total = (long)d.Bc * d.N * d.K * d.N;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int o = idx % d.N;
+    const int k = (idx / d.N) % d.K;
+    const int i = (idx / ((long)d.N * d.K)) % d.N;
+    const int w = idx / ((long)d.N * d.K * d.N);
+    float* out = env + idx * 5;
+    if (kind == 2) {  // null envelope
+      out[0] = 1.f;
+      out[1] = out[2] = out[3] = out[4] = 0.f;
+      continue;
+    }
+    const bool up = d.nch == 1 || i < d.n_up;
+    const float* pi = up ? pi0 : pi1;
+    const float* sg = up ? sg0 : sg1;
+    const float* xi = elec + ((long)w * d.N + i) * 3;
+    float v = 0.f, l = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int I = 0; I < d.M; ++I) {
+      const float dx = xi[0] - atoms[3 * I], dy = xi[1] - atoms[3 * I + 1], dz = xi[2] - atoms[3 * I + 2];
+      const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
+      const long pidx = ((long)o * d.M + I) * d.K + k;
+      const float a = kind == 1 ? fabsf(sg[pidx]) : sg[pidx];
+      const float f = pi[pidx] * expf(-a * rr);
+      v += f;
+      l += f * (a * a - 2.f * a / rr);
+      const float gs = -a * f / rr;
+      gx += gs * dx;
+      gy += gs * dy;
+      gz += gs * dz;
+    }
+    out[0] = v;
+    out[1] = l;
+    out[2] = gx;
+    out[3] = gy;
+    out[4] = gz;
+  }
+}
+
+inline int grid_for(long total) {
+  long g = (total + 255) / 256;
+  if (g > 65536L * 8) g = 65536L * 8;
+  return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace
+
+int launch_ferminet_features(const FermiDims& d, const float* elec, const float* atoms, void* X, long ldx,
+                             bool x_bf16, float* H2, hipStream_t s) {
+  const long t1 = (long)d.Bc * d.N * d.Rp * 4 * d.M;
+  if (x_bf16)
+    hipLaunchKernelGGL(ae_features_kernel<bf16>, grid_for(t1), 256, 0, s, d, elec, atoms, (bf16*)X, ldx);
+  else
+    hipLaunchKernelGGL(ae_features_kernel<float>, grid_for(t1), 256, 0, s, d, elec, atoms, (float*)X, ldx);
+  const long t2 = (long)d.Bc * d.N * d.N;
+  hipLaunchKernelGGL(ee_features_kernel, grid_for(t2), 256, 0, s, d, d.Rp > 1 ? 8 : 1, elec, H2);
+  return 0;
+}
+
+int launch_pair_layer(const FermiDims& d, const float* H2in, int fin, float* H2out, int fout, const float* W,
+                      const float* b, bool residual, hipStream_t s) {
+  const long npairs = (long)d.Bc * d.N * d.N;
+  const long total = npairs * fout;
+  if (d.Rp > 1)
+    hipLaunchKernelGGL(pair_layer_kernel<8>, grid_for(total), 256, 0, s, npairs, H2in, fin, H2out, fout, W, b,
+                       residual ? 1 : 0);
+  else
+    hipLaunchKernelGGL(pair_layer_kernel<1>, grid_for(total), 256, 0, s, npairs, H2in, fin, H2out, fout, W, b,
+                       residual ? 1 : 0);
+  return 0;
+}
+
+int launch_pair_means(const FermiDims& d, const float* H2, int f2, void* X, long ldx, int col0, bool x_bf16,
+                      hipStream_t s) {
+  const long total = (long)d.Bc * d.N * d.Rp * (ldx - col0);
+  const int RP2 = d.Rp > 1 ? 8 : 1;
+  if (x_bf16)
+    hipLaunchKernelGGL(pair_means_kernel<bf16>, grid_for(total), 256, 0, s, d, RP2, H2, f2, (bf16*)X, ldx, col0);
+  else
+    hipLaunchKernelGGL(pair_means_kernel<float>, grid_for(total), 256, 0, s, d, RP2, H2, f2, (float*)X, ldx, col0);
+  return 0;
+}
+
+int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void* g, long ldg, bool bf16io,
+                      hipStream_t s) {
+  const long total = (long)d.Bc * d.Rp * ldg;
+  if (bf16io)
+    hipLaunchKernelGGL(spin_means_kernel<bf16>, grid_for(total), 256, 0, s, d, (const bf16*)X, ldx, f1, (bf16*)g,
+                       ldg);
+  else
+    hipLaunchKernelGGL(spin_means_kernel<float>, grid_for(total), 256, 0, s, d, (const float*)X, ldx, f1,
+                       (float*)g, ldg);
+  return 0;
+}
+
+int launch_envelope(const FermiDims& d, const float* elec, const float* atoms, const float* const* pi,
+                    const float* const* sigma, int env_kind, float* env, hipStream_t s) {
+  const long total = (long)d.Bc * d.N * d.K * d.N;
+  hipLaunchKernelGGL(envelope_kernel, grid_for(total), 256, 0, s, d, elec, atoms, pi[0], sigma[0],
+                     d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
+  return 0;
+}

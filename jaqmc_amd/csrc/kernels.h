@@ -1,0 +1,85 @@
+// Host-side launchers of the gfx950 kernels (one declaration per kernel family).
+#pragma once
+
+#include "common.h"
+
+enum { PREC_F32 = 0, PREC_BF16 = 1, PREC_BF16X3 = 2 };
+enum { ACT_NONE = 0, ACT_TANH = 1 };
+
+// Y[rows, ncols] = epilogue(X[rows, K] . Wt[ncols, K]^T)
+//
+// Operands are K-contiguous ("NT"): X rows and Wt rows (one per output
+// column).  K must be a multiple of 32 and ldx, ldw multiples of 8; padding
+// columns of X and Wt must be zero.  In Laplacian mode (R > 1) rows come in
+// units of R = Rp tangent rows (value, Laplacian, Jacobian...) and one
+// workgroup owns one whole unit, so the elementwise forward-Laplacian rule of
+// the activation can be applied in the epilogue:
+//   y   = acc + G[walker(unit), r] + bias * [r == value]
+//   tanh: value -> t, Jacobian -> (1-t^2) J, Laplacian -> (1-t^2) L - 2t(1-t^2) sum_c J_c^2
+//   residual: out = (out + Res[row]) * res_scale
+struct GemmArgs {
+  const void* X; long ldx;
+  const void* Wt; const void* Wt_lo; long ldw;  // Wt_lo: low BF16 half for PREC_BF16X3
+  void* Y; long ldy;
+  int rows, K, ncols;
+  const float* bias;   // (ncols) or null
+  const float* G; long ldg;  // per-walker broadcast term or null
+  int R;               // rows per unit (Rp) in Laplacian mode, 1 otherwise
+  int act;             // ACT_*
+  const void* Res; long ldres;  // residual source (same element type as Y) or null
+  float res_scale;
+  int Bc, n_up, n_dn;  // unit -> walker map (spin-blocked units)
+};
+
+// prec selects the MFMA path and the element type of X (bf16 for PREC_BF16,
+// float otherwise); y_bf16 selects the element type of Y/Res.
+int launch_gemm(const GemmArgs& a, int prec, bool y_bf16, hipStream_t s);
+
+// ---- FermiNet feature / stream kernels (ferminet.hip) ----
+struct FermiDims {
+  int Bc, N, n_up, n_dn, M, K;  // walkers in chunk, electrons, atoms, dets
+  int Rp;                       // rows per unit (1 in forward mode)
+  int nch;                      // non-empty spin channels (1 or 2)
+};
+// Layer-0 one-electron input (ae features) into X[:, :, 0:4M] and the pair
+// stream input H2 (RP2 rows per pair: 8 in Laplacian mode, 1 in forward mode).
+int launch_ferminet_features(const FermiDims& d, const float* elec, const float* atoms,
+                             void* X, long ldx, bool x_bf16, float* H2, hipStream_t s);
+// One pair-stream layer: H2out = tanh(H2in W + b) (+ residual).
+int launch_pair_layer(const FermiDims& d, const float* H2in, int fin, float* H2out, int fout,
+                      const float* W, const float* b, bool residual, hipStream_t s);
+// Means over the first electron index of the pair stream, expanded to dense
+// tangent rows, written into X[:, :, col0 : col0 + nch*f2].
+int launch_pair_means(const FermiDims& d, const float* H2, int f2, void* X, long ldx, int col0,
+                      bool x_bf16, hipStream_t s);
+// Spin-channel means of X[:, :, 0:f1] -> g[walker, r, ch*f1 + f]  (ldg row stride).
+int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void* g, long ldg,
+                      bool bf16io, hipStream_t s);
+// Envelope value / Laplacian / gradient: env[walker, i, k*N + o, 5].
+int launch_envelope(const FermiDims& d, const float* elec, const float* atoms, const float* const* pi,
+                    const float* const* sigma, int env_kind, float* env, hipStream_t s);
+
+// ---- determinants (det.hip) ----
+// Per (walker, det): FP64 Gauss-Jordan inverse of A = Phi * env, log|det|, sign,
+// and (Laplacian mode) d logdet / dx_c and the Laplacian of log|det|.
+// out layout per (w, k): [logdet, sign, lap, grad[3N]] doubles (stride 3 + 3N).
+int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* env, double* out,
+               hipStream_t s);
+// Combine determinants: log|psi| (+ Jastrow for Psiformer), E_kin, V, E_L.
+struct CombineArgs {
+  const double* det; int B_off;  // chunk offset into the caller's arrays
+  const float* elec; const float* atoms; const float* charges;
+  int jastrow; float alpha_par, alpha_anti;
+  float* logpsi; float* sign; double* e_kin; double* e_pot; double* e_loc; float* grad;
+  int lapmode;
+};
+int launch_combine(const FermiDims& d, const CombineArgs& a, hipStream_t s);
+
+// ---- Metropolis (mcmc.hip) ----
+int launch_propose(const float* elec, float* prop, int B, int N, float width, const float* normals,
+                   uint64_t seed, uint64_t ctr, hipStream_t s);
+int launch_accept(float* elec, const float* prop, float* logp, const float* logpsi_prop, int B, int N,
+                  const float* uniforms, uint64_t seed, uint64_t ctr, int32_t* n_accept, hipStream_t s);
+int launch_reduce_mean_var(const double* x, int B, double* out, hipStream_t s);
+int launch_reduce_accepts(const int32_t* x, int B, double* out, hipStream_t s);
+int launch_scale_f32(float* x, int n, float a, hipStream_t s);

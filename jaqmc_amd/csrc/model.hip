@@ -1,0 +1,327 @@
+// Model construction (parameter pack -> device layouts) and the per-chunk
+// pipelines shared by log|psi| and the local energy.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "model.h"
+
+int gemm_unit_rows(int R);
+size_t det_smem_bytes(int N, int* Np_out, int* cpi_out);
+
+namespace {
+
+struct Reader {
+  const float* p;
+  size_t n, pos = 0;
+  bool ok = true;
+  const float* take(size_t k) {
+    if (pos + k > n) {
+      ok = false;
+      return nullptr;
+    }
+    const float* r = p + pos;
+    pos += k;
+    return r;
+  }
+};
+
+// Transpose rows [r0, r0 + nr) of a row-major (rows x ncols) matrix into
+// dst (ncols x kpad) at column offset c0.
+void put_t(std::vector<float>& dst, int kpad, const float* W, int ncols, int r0, int nr, int c0) {
+  for (int r = 0; r < nr; ++r)
+    for (int n = 0; n < ncols; ++n) dst[(size_t)n * kpad + c0 + r] = W[(size_t)(r0 + r) * ncols + n];
+}
+
+constexpr size_t kDefaultBudget = 24ull << 30;  // bytes of walker-chunk workspace
+
+}  // namespace
+
+int jq_model::init(const jq_model_desc& desc, const float* packed, size_t n_floats) {
+  d = desc;
+  kind = desc.kind;
+  n_up = desc.n_up;
+  n_dn = desc.n_dn;
+  N = n_up + n_dn;
+  T = 3 * N;
+  M = desc.n_atoms;
+  K = desc.ndets;
+  prec = desc.precision;
+  xbf = prec == JQ_PREC_BF16;
+  nch = (n_up > 0 && n_dn > 0) ? 2 : 1;
+  if (N < 1 || N > 63 || M < 1 || K < 1 || n_up < 0 || n_dn < 0)
+    return jq_fail(JQ_EINVAL, "unsupported system size (1 <= N <= 63 electrons, >= 1 atom, >= 1 det)");
+  if (prec < 0 || prec > 2) return jq_fail(JQ_EINVAL, "bad precision");
+  if (desc.envelope < 0 || desc.envelope > 2) return jq_fail(JQ_EINVAL, "bad envelope kind");
+  if (!desc.atoms || !desc.charges) return jq_fail(JQ_EINVAL, "atoms / charges missing");
+  Rp = gemm_unit_rows(T + 2);
+  if (Rp < 0) return jq_fail(JQ_EINVAL, "too many electrons for the tangent tile");
+  atoms_h.assign(desc.atoms, desc.atoms + 3 * M);
+  charges_h.assign(desc.charges, desc.charges + M);
+  int rc = upload(atoms_h.data(), 12 * M, (void**)&atoms);
+  if (rc) return rc;
+  rc = upload(charges_h.data(), 4 * M, (void**)&charges);
+  if (rc) return rc;
+  if (kind == JQ_KIND_FERMINET) return init_ferminet(packed, n_floats);
+  if (kind == JQ_KIND_PSIFORMER) return init_psiformer(packed, n_floats);
+  return jq_fail(JQ_EINVAL, "unknown model kind");
+}
+
+// Output layers shared by both ansatzes: spin-split orbital projection
+// (orbital.py:60-78) and per-channel envelope parameters (envelope.py:70-96).
+static int read_outputs(jq_model* m, Reader& rd, int d_last) {
+  m->korb = round_up(d_last, 32);
+  const int ncol = m->K * m->N;
+  for (int c = 0; c < m->nch; ++c) {
+    const float* W = rd.take((size_t)d_last * ncol);
+    if (!W) return jq_fail(JQ_EINVAL, "parameter pack too short (orbital)");
+    std::vector<float> wt((size_t)ncol * m->korb, 0.f);
+    put_t(wt, m->korb, W, ncol, 0, d_last, 0);
+    int rc = m->upload_wt(wt, &m->Worb[c], &m->Worb_lo[c]);
+    if (rc) return rc;
+  }
+  const size_t ne = (size_t)m->N * m->M * m->K;
+  for (int c = 0; c < m->nch; ++c) {
+    const float* pi = rd.take(ne);
+    const float* sg = rd.take(ne);
+    if (!pi || !sg) return jq_fail(JQ_EINVAL, "parameter pack too short (envelope)");
+    int rc = m->upload(pi, ne * 4, (void**)&m->pi[c]);
+    if (rc) return rc;
+    rc = m->upload(sg, ne * 4, (void**)&m->sigma[c]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int jq_model::init_ferminet(const float* packed, size_t n_floats) {
+  L = d.n_layers;
+  if (L < 1 || L > JQ_MAX_LAYERS) return jq_fail(JQ_EINVAL, "bad n_layers");
+  Reader rd{packed, n_floats};
+  fl.resize(L);
+  int f1 = 4 * M, f2 = 4;
+  std::vector<const float*> Ws(L), W2s(L), bs(L), b2s(L);
+  for (int l = 0; l < L; ++l) {
+    FermiLayer& ly = fl[l];
+    ly.f1 = f1;
+    ly.f2 = f2;
+    ly.d1 = d.hidden_single[l];
+    ly.d2 = d.hidden_double[l];
+    ly.has_pair = l < L - 1;
+    ly.res1 = f1 == ly.d1;
+    ly.res2 = ly.has_pair && f2 == ly.d2;
+    ly.kmain = round_up(f1 + nch * f2, 32);
+    ly.kg = round_up(nch * f1, 32);
+    if (ly.d1 < 1 || ly.d2 < 1) return jq_fail(JQ_EINVAL, "bad hidden width");
+    const int in = f1 * (1 + nch) + f2 * nch;
+    Ws[l] = rd.take((size_t)in * ly.d1);
+    bs[l] = rd.take(ly.d1);
+    f1 = ly.d1;
+    if (ly.has_pair) f2 = ly.d2;
+  }
+  for (int l = 0; l < L - 1; ++l) {
+    W2s[l] = rd.take((size_t)fl[l].f2 * fl[l].d2);
+    b2s[l] = rd.take(fl[l].d2);
+  }
+  if (!rd.ok) return jq_fail(JQ_EINVAL, "parameter pack too short (FermiNet layers)");
+  for (int l = 0; l < L; ++l) {
+    FermiLayer& ly = fl[l];
+    // W rows: [h (f1) | g_one (nch*f1) | g_two (nch*f2)]  (aggregate_features order)
+    std::vector<float> wm((size_t)ly.d1 * ly.kmain, 0.f), wg((size_t)ly.d1 * ly.kg, 0.f);
+    put_t(wm, ly.kmain, Ws[l], ly.d1, 0, ly.f1, 0);
+    put_t(wm, ly.kmain, Ws[l], ly.d1, ly.f1 * (1 + nch), nch * ly.f2, ly.f1);
+    put_t(wg, ly.kg, Ws[l], ly.d1, ly.f1, nch * ly.f1, 0);
+    int rc = upload_wt(wm, &ly.Wm, &ly.Wm_lo);
+    if (!rc) rc = upload_wt(wg, &ly.Wg, &ly.Wg_lo);
+    if (!rc) rc = upload(bs[l], 4 * ly.d1, (void**)&ly.b);
+    if (!rc && ly.has_pair) {
+      rc = upload(W2s[l], 4L * ly.f2 * ly.d2, (void**)&ly.W2);
+      if (!rc) rc = upload(b2s[l], 4L * ly.d2, (void**)&ly.b2);
+    }
+    if (rc) return rc;
+  }
+  const int d_last = fl[L - 1].d1;
+  if (d_last % 32 != 0) return jq_fail(JQ_EINVAL, "last one-electron width must be a multiple of 32");
+  int rc = read_outputs(this, rd, d_last);
+  if (rc) return rc;
+  if (rd.pos != n_floats) return jq_fail(JQ_EINVAL, "parameter pack size mismatch");
+
+  // ---- workspace ----
+  size_t kx = 0, kgm = 0, d1m = 0, f2m = 4;
+  for (auto& ly : fl) {
+    kx = std::max(kx, (size_t)ly.kmain);
+    kgm = std::max(kgm, (size_t)ly.kg);
+    d1m = std::max(d1m, (size_t)ly.d1);
+    f2m = std::max(f2m, (size_t)ly.d2);
+  }
+  kx = std::max(kx, (size_t)korb);
+  ldphi = round_up(K * N, 4);
+  auto per_walker = [&](int R, int RP2) {
+    return 2 * (size_t)N * R * kx * es() + (size_t)R * kgm * es() + (size_t)R * d1m * 4 +
+           2 * (size_t)N * N * RP2 * f2m * 4 + (size_t)N * R * ldphi * 4 + (size_t)N * K * N * 5 * 4 +
+           (size_t)K * (3 + T) * 8 + 64;
+  };
+  const size_t pw_lap = per_walker(Rp, 8), pw_fwd = per_walker(1, 1);
+  chunk = d.chunk_walkers > 0 ? d.chunk_walkers : (int)std::max<size_t>(1, kDefaultBudget / pw_lap);
+  chunk = std::min(chunk, 1 << 16);
+  chunk_fwd = (int)std::min<size_t>((size_t)chunk * pw_lap / pw_fwd, 1 << 20);
+  const size_t nX = std::max((size_t)chunk * N * Rp * kx, (size_t)chunk_fwd * N * kx);
+  const size_t ng = std::max((size_t)chunk * Rp * kgm, (size_t)chunk_fwd * kgm);
+  const size_t nG = std::max((size_t)chunk * Rp * d1m, (size_t)chunk_fwd * d1m);
+  const size_t nH = std::max((size_t)chunk * N * N * 8 * f2m, (size_t)chunk_fwd * N * N * f2m);
+  const size_t nP = std::max((size_t)chunk * N * Rp * ldphi, (size_t)chunk_fwd * N * ldphi);
+  const size_t nE = (size_t)std::max(chunk, chunk_fwd) * N * K * N * 5;
+  const size_t nD = std::max((size_t)chunk * K * (3 + T), (size_t)chunk_fwd * K * 3);
+  X[0] = alloc(nX * es());
+  X[1] = alloc(nX * es());
+  g = alloc(ng * es());
+  G = (float*)alloc(nG * 4);
+  H2[0] = (float*)alloc(nH * 4);
+  H2[1] = (float*)alloc(nH * 4);
+  Phi = (float*)alloc(nP * 4);
+  env = (float*)alloc(nE * 4);
+  det = (double*)alloc(nD * 8);
+  if (!X[0] || !X[1] || !g || !G || !H2[0] || !H2[1] || !Phi || !env || !det)
+    return jq_fail(JQ_ENOMEM, "workspace allocation failed");
+  return 0;
+}
+
+int jq_model::ensure_scratch(int B) {
+  if (B <= prop_cap) return 0;
+  const int cap = std::max(B, 1024);
+  float* p = nullptr;
+  float* q = nullptr;
+  if (hipMalloc(&p, (size_t)cap * T * 4) != hipSuccess || hipMalloc(&q, (size_t)cap * 4) != hipSuccess)
+    return jq_fail(JQ_ENOMEM, "Metropolis scratch allocation failed");
+  if (prop) {
+    (void)hipFree(prop);
+    (void)hipFree(lp);
+    allocs.erase(std::remove(allocs.begin(), allocs.end(), (void*)prop), allocs.end());
+    allocs.erase(std::remove(allocs.begin(), allocs.end(), (void*)lp), allocs.end());
+  }
+  prop = p;
+  lp = q;
+  allocs.push_back(p);
+  allocs.push_back(q);
+  prop_cap = cap;
+  return 0;
+}
+
+// ---- FermiNet pipeline --------------------------------------------------------
+int jq_model::forward_ferminet(const float* elec, int Bc, bool lap, hipStream_t s) {
+  const FermiDims fd = dims(Bc, lap);
+  const int RP = fd.Rp;
+  const int gp = lap ? PREC_F32 : PREC_F32;
+  (void)gp;
+  void* Xc = X[0];
+  void* Xn = X[1];
+  int rc = launch_ferminet_features(fd, elec, atoms, Xc, fl[0].kmain, xbf, H2[0], s);
+  int hc = 0;
+  for (int l = 0; l < L && !rc; ++l) {
+    const FermiLayer& ly = fl[l];
+    rc = launch_pair_means(fd, H2[hc], ly.f2, Xc, ly.kmain, ly.f1, xbf, s);
+    if (!rc) rc = launch_spin_means(fd, Xc, ly.kmain, ly.f1, g, ly.kg, xbf, s);
+    if (rc) break;
+    GemmArgs ga{};
+    ga.X = g;
+    ga.ldx = ly.kg;
+    ga.Wt = ly.Wg;
+    ga.Wt_lo = ly.Wg_lo;
+    ga.ldw = ly.kg;
+    ga.Y = G;
+    ga.ldy = ly.d1;
+    ga.rows = Bc * RP;
+    ga.K = ly.kg;
+    ga.ncols = ly.d1;
+    ga.R = 1;
+    ga.act = ACT_NONE;
+    rc = launch_gemm(ga, prec, false, s);
+    if (rc) return jq_fail(JQ_EINVAL, "spin-mean GEMM launch rejected");
+    const int ldn = (l + 1 < L) ? fl[l + 1].kmain : korb;
+    GemmArgs gm{};
+    gm.X = Xc;
+    gm.ldx = ly.kmain;
+    gm.Wt = ly.Wm;
+    gm.Wt_lo = ly.Wm_lo;
+    gm.ldw = ly.kmain;
+    gm.Y = Xn;
+    gm.ldy = ldn;
+    gm.rows = Bc * N * RP;
+    gm.K = ly.kmain;
+    gm.ncols = ly.d1;
+    gm.bias = ly.b;
+    gm.G = G;
+    gm.ldg = ly.d1;
+    gm.R = RP;
+    gm.act = ACT_TANH;
+    gm.Res = ly.res1 ? Xc : nullptr;
+    gm.ldres = ly.kmain;
+    gm.res_scale = 0.70710678118654752f;
+    gm.Bc = Bc;
+    gm.n_up = n_up;
+    gm.n_dn = n_dn;
+    rc = launch_gemm(gm, prec, xbf, s);
+    if (rc) return jq_fail(JQ_EINVAL, "layer GEMM launch rejected");
+    if (ly.has_pair) {
+      rc = launch_pair_layer(fd, H2[hc], ly.f2, H2[hc ^ 1], ly.d2, ly.W2, ly.b2, ly.res2, s);
+      hc ^= 1;
+    }
+    std::swap(Xc, Xn);
+  }
+  if (rc) return jq_fail(JQ_EHIP, "FermiNet kernel launch failed");
+  return orbitals_and_dets(elec, Xc, Bc, lap, s);
+}
+
+// Orbital projection per spin channel, envelope, determinants.
+int jq_model::orbitals_and_dets(const float* elec, const void* Xlast, int Bc, bool lap, hipStream_t s) {
+  const FermiDims fd = dims(Bc, lap);
+  const int RP = fd.Rp;
+  for (int c = 0; c < nch; ++c) {
+    const long u0 = (nch == 1 || c == 0) ? 0 : (long)Bc * n_up;
+    const long nu = nch == 1 ? (long)Bc * N : (c == 0 ? (long)Bc * n_up : (long)Bc * n_dn);
+    GemmArgs go{};
+    go.X = (const char*)Xlast + u0 * RP * korb * es();
+    go.ldx = korb;
+    go.Wt = Worb[c];
+    go.Wt_lo = Worb_lo[c];
+    go.ldw = korb;
+    go.Y = Phi + u0 * RP * ldphi;
+    go.ldy = ldphi;
+    go.rows = (int)(nu * RP);
+    go.K = korb;
+    go.ncols = K * N;
+    go.R = 1;
+    go.act = ACT_NONE;
+    if (launch_gemm(go, prec, false, s)) return jq_fail(JQ_EINVAL, "orbital GEMM launch rejected");
+  }
+  const float* pis[2] = {pi[0], pi[1]};
+  const float* sgs[2] = {sigma[0], sigma[1]};
+  launch_envelope(fd, elec, atoms, pis, sgs, d.envelope, env, s);
+  if (launch_det(fd, Phi, ldphi, env, det, s)) return jq_fail(JQ_EINVAL, "determinant launch rejected");
+  return 0;
+}
+
+int jq_model::forward(const float* elec, int b0, int Bc, bool lap, float* logpsi, float* sign, double* e_kin,
+                      double* e_pot, double* e_loc, float* grad, hipStream_t s) {
+  const float* ec = elec + (long)b0 * T;
+  int rc = kind == JQ_KIND_FERMINET ? forward_ferminet(ec, Bc, lap, s) : forward_psiformer(ec, Bc, lap, s);
+  if (rc) return rc;
+  CombineArgs ca{};
+  ca.det = det;
+  ca.B_off = b0;
+  ca.elec = elec;
+  ca.atoms = atoms;
+  ca.charges = charges;
+  ca.jastrow = (kind == JQ_KIND_PSIFORMER && d.jastrow) ? 1 : 0;
+  ca.alpha_par = jas[0];
+  ca.alpha_anti = jas[1];
+  ca.logpsi = logpsi;
+  ca.sign = sign;
+  ca.e_kin = e_kin;
+  ca.e_pot = e_pot;
+  ca.e_loc = e_loc;
+  ca.grad = grad;
+  ca.lapmode = lap ? 1 : 0;
+  launch_combine(dims(Bc, lap), ca, s);
+  return 0;
+}

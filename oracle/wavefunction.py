@@ -1,0 +1,285 @@
+"""CPU oracle: FermiNet and Psiformer log|psi| with dense forward Laplacian.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Every function restates the reference's single-walker Flax module as a
+batched numpy computation on :class:`oracle.lap.Lap` values, so one code path
+yields log|psi| (no tangents) or (log|psi|, grad, Laplacian) (3N tangents).
+
+Parameter dictionaries use the flat layout of ``jaqmc_amd.params`` (plain
+numpy arrays, kernels stored as (fan_in, fan_out)).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .lap import Lap, concat, slogdet
+
+SQRT2 = np.sqrt(2.0)
+
+
+def spin_channels(nspins):
+    """Non-empty spin channels as (start, stop) ranges.
+
+    Restates utils/array.py:24-45 (``split_nonempty_channels``): zero-sized
+    channels are dropped, and with <= 1 non-empty channel the whole electron
+    axis is a single channel.
+    """
+    n_up, n_dn = nspins
+    if (n_up > 0) + (n_dn > 0) <= 1:
+        return [(0, n_up + n_dn)]
+    return [(0, n_up), (n_up, n_up + n_dn)]
+
+
+# --------------------------------------------------------------------------
+# geometry / input features
+# --------------------------------------------------------------------------
+def pair_within(x: Lap):
+    """r_i - r_j and |r_i - r_j| with the +eye diagonal trick (geometry/obc.py:39-48)."""
+    N = x.shape[1]
+    disp = x.expand(2) - x.expand(1)  # (B, N, N, 3)
+    eye = np.eye(N, dtype=x.dtype)
+    r = (disp + eye[..., None]).square().sum(-1).sqrt() * (1.0 - eye)
+    return disp, r
+
+
+def pair_between(x: Lap, atoms):
+    """r_i - R_I and |r_i - R_I| (geometry/obc.py:86-87)."""
+    disp = x.expand(2) - np.asarray(atoms, x.dtype)  # (B, N, M, 3)
+    r = disp.square().sum(-1).sqrt()
+    return disp, r
+
+
+def molecule_features(x: Lap, atoms, rescale: bool):
+    """MoleculeFeatures (wavefunction/input/atomic.py:46-79)."""
+    B, N = x.shape[:2]
+    ee_vec, r_ee = pair_within(x)
+    ae_vec, r_ae = pair_between(x, atoms)
+    if rescale:
+        log_r_ae = r_ae.log1p().expand(3)
+        ae = concat([log_r_ae, ae_vec * log_r_ae / r_ae.expand(3)], -1)
+        # (ee features with rescale divide 0/0 on the diagonal; Psiformer never uses them)
+        ee = None
+    else:
+        ae = concat([r_ae.expand(3), ae_vec], -1)
+        ee = concat([r_ee.expand(3), ee_vec], -1)
+    M = ae.shape[2]
+    ae = ae.reshape(N, 4 * M)
+    return dict(ae_features=ae, ee_features=ee, r_ae=r_ae, ae_vec=ae_vec, r_ee=r_ee)
+
+
+# --------------------------------------------------------------------------
+# FermiNet backbone
+# --------------------------------------------------------------------------
+def aggregate(h1: Lap, h2: Lap, nspins):
+    """FermiLayers.aggregate_features (wavefunction/backbone/ferminet.py:65-90).
+
+    [h_one, spin means of h_one (broadcast), means over the FIRST electron
+    index of h_two per spin channel].
+    """
+    parts = [h1]
+    chans = spin_channels(nspins)
+    for (a, b) in chans:
+        parts.append(h1[a:b].mean(1).expand(1).broadcast_to(h1.shape[1:]))
+    for (a, b) in chans:
+        parts.append(h2[a:b].mean(1))
+    return concat(parts, -1)
+
+
+def ferminet_backbone(params, feats, nspins):
+    """FermiLayers.__call__ with use_last_layer=False (ferminet.py:31-55)."""
+    h1, h2 = feats["ae_features"], feats["ee_features"]
+    L = len(params["single"])
+    for l in range(L):
+        hin = aggregate(h1, h2, nspins)
+        W, b = params["single"][l]
+        new = hin.matmul(W, b).tanh()
+        h1 = (h1 + new) / SQRT2 if new.shape == h1.shape else new
+        if l < L - 1:
+            W2, b2 = params["double"][l]
+            new2 = h2.matmul(W2, b2).tanh()
+            h2 = (h2 + new2) / SQRT2 if new2.shape == h2.shape else new2
+    return h1
+
+
+# --------------------------------------------------------------------------
+# Psiformer backbone
+# --------------------------------------------------------------------------
+def layer_norm(x: Lap, scale, bias, eps=1e-5):
+    """LayerNorm over the feature axis (flax nn.LayerNorm, epsilon=1e-5)."""
+    mu = x.mean(-1).expand(-1)
+    d = x - mu
+    var = d.square().mean(-1).expand(-1)
+    return d * (var + eps).rsqrt() * scale + bias
+
+
+def bilinear(x: Lap, y: Lap, spec: str):
+    """einsum of two coordinate-dependent operands with the product rule.
+
+    ``spec`` is written for the value arrays (batch-first, e.g.
+    'bihd,bjhd->bhij'); Jacobians get a tangent axis 't' after 'b'.
+    """
+    ins, out = spec.split("->")
+    a, c = ins.split(",")
+    tj = lambda s: s[0] + "t" + s[1:]
+    v = np.einsum(spec, x.v, y.v, optimize=True)
+    J = np.einsum(f"{tj(a)},{c}->{tj(out)}", x.J, y.v, optimize=True) + \
+        np.einsum(f"{a},{tj(c)}->{tj(out)}", x.v, y.J, optimize=True)
+    lap = np.einsum(spec, x.lap, y.v, optimize=True) + np.einsum(spec, x.v, y.lap, optimize=True) + \
+        2.0 * np.einsum(f"{tj(a)},{tj(c)}->{out}", x.J, y.J, optimize=True)
+    return Lap(v, J, lap)
+
+
+def mha(x: Lap, p, num_heads):
+    """flax MultiHeadDotProductAttention, self-attention, deterministic.
+
+    q, k, v = DenseGeneral(D -> (H, dh)) with bias; q scaled by 1/sqrt(dh);
+    softmax over keys; output DenseGeneral((H, dh) -> D) with bias
+    (wavefunction/backbone/psiformer.py:76-82).
+    """
+    B, N, D = x.shape
+    H = num_heads
+    dh = p["wq"].shape[1] // H
+    q = x.matmul(p["wq"], p["bq"]).reshape(N, H, dh) * (1.0 / np.sqrt(dh))
+    k = x.matmul(p["wk"], p["bk"]).reshape(N, H, dh)
+    v = x.matmul(p["wv"], p["bv"]).reshape(N, H, dh)
+    s = bilinear(q, k, "bihd,bjhd->bhij")
+    s = s - s.v.max(-1, keepdims=True)
+    e = s.exp()
+    w = e * e.sum(-1).expand(-1).recip()
+    o = bilinear(w, v, "bhij,bjhd->bihd").reshape(N, H * dh)
+    return o.matmul(p["wo"], p["bo"])
+
+
+def psiformer_backbone(params, feats, nspins, num_heads):
+    """PsiformerBackbone with pre-LN layers (wavefunction/backbone/psiformer.py:64-187)."""
+    h = feats["ae_features"]
+    B, N = h.shape[:2]
+    n_up, n_dn = nspins
+    spins = np.concatenate([np.ones(n_up), -np.ones(n_dn)]).astype(h.dtype)[:, None]
+    h = concat([h, Lap.const(spins, B, h.T, h.dtype)], -1)
+    x = h.matmul(params["input"][0], params["input"][1])
+    for lp in params["layers"]:
+        xin = layer_norm(x, lp["ln1_scale"], lp["ln1_bias"])
+        x = x + mha(xin, lp, num_heads)
+        m = layer_norm(x, lp["ln2_scale"], lp["ln2_bias"])
+        for (W, b) in lp["mlp"]:
+            m = m.matmul(W, b).tanh()
+        x = x + m
+    return x
+
+
+def jastrow(r_ee: Lap, nspins, alpha_par, alpha_anti):
+    """SimpleEEJastrow (wavefunction/jastrow.py:60-122): sum over pairs of -c a^2/(a+r)."""
+    n_up, n_dn = nspins
+    N = n_up + n_dn
+    spin = np.array([0] * n_up + [1] * n_dn)
+    iu = np.triu(np.ones((N, N)), 1)
+    same = (spin[:, None] == spin[None, :]) * iu
+    anti = (spin[:, None] == 0) & (spin[None, :] == 1)
+    f_par = (r_ee + alpha_par).recip() * (-0.25 * alpha_par ** 2)
+    f_anti = (r_ee + alpha_anti).recip() * (-0.5 * alpha_anti ** 2)
+    total = (f_par * same.astype(r_ee.dtype)).sum(-1).sum(-1) + (f_anti * anti.astype(r_ee.dtype)).sum(-1).sum(-1)
+    return total
+
+
+# --------------------------------------------------------------------------
+# orbitals, envelope, determinants
+# --------------------------------------------------------------------------
+def orbitals(params, h: Lap, nspins, ndets):
+    """OrbitalProjection with spin split (wavefunction/output/orbital.py:60-78).
+
+    Returns (B, K, N, N): [det, electron, orbital].
+    """
+    N = sum(nspins)
+    chans = spin_channels(nspins)
+    blocks = []
+    for c, (a, b) in enumerate(chans):
+        W = params["orbital"][c]
+        phi = h[a:b].matmul(W, params.get("orbital_bias", [None] * len(chans))[c])
+        blocks.append(phi.reshape(b - a, ndets, N))
+    phi = concat(blocks, 1) if len(blocks) > 1 else blocks[0]
+    return phi.transpose(1, 0, 2)
+
+
+def envelope(params, r_ae: Lap, nspins, ndets, kind):
+    """Envelope (wavefunction/output/envelope.py:38-140), per spin channel.
+
+    env[k, i, o] = sum_I pi[o, I, k] exp(-|sigma[o, I, k] r_iI|)  (abs_isotropic)
+    or exp(-sigma r) (isotropic); 'null' gives ones.
+    """
+    B, N, M = r_ae.shape
+    if kind == "null":
+        return None
+    chans = spin_channels(nspins)
+    blocks = []
+    for c, (a, b) in enumerate(chans):
+        pi, sigma = params["envelope"][c]  # (N_orb, M, K)
+        r = r_ae[a:b].expand(2).expand(4)  # (B, n, 1, M, 1)
+        sr = r * sigma
+        expo = -(sr.abs()) if kind == "abs_isotropic" else -sr
+        env = (expo.exp() * pi).sum(3)  # (B, n, N_orb, K)
+        blocks.append(env.transpose(2, 0, 1))  # (B, K, n, N_orb)
+    return concat(blocks, 2) if len(blocks) > 1 else blocks[0]
+
+
+def logdet_combine(A: Lap):
+    """LogDet (wavefunction/output/logdet.py:40-79) with the Laplacian of log|sum_k s_k det_k|.
+
+    The max shift m is treated as a constant: the result is invariant to it.
+    """
+    sign_k, ld = slogdet(A)
+    m = ld.v.max(-1, keepdims=True)
+    S = ((ld - m).exp() * sign_k).sum(-1)
+    sgn = np.sign(S.v)
+    logpsi = (S * sgn).log() + m[:, 0]
+    return sgn, logpsi, sign_k, ld
+
+
+# --------------------------------------------------------------------------
+# full wavefunctions
+# --------------------------------------------------------------------------
+def logpsi(model, params, x, atoms, with_laplacian=True):
+    """log|psi| (and, with_laplacian, its 3N-gradient and Laplacian).
+
+    model: dict(kind='ferminet'|'psiformer', nspins, ndets, envelope, num_heads).
+    Returns (sign, Lap) where Lap.v is (B,), Lap.J is (B, 3N), Lap.lap is (B,).
+    FermiNet: app/molecule/wavefunction/ferminet.py:76-96.
+    Psiformer: app/molecule/wavefunction/psiformer.py:136-166.
+    """
+    xl = Lap.coords(np.asarray(x), with_tangents=with_laplacian)
+    B, N, _ = xl.shape
+    nspins = tuple(model["nspins"])
+    K = model["ndets"]
+    if model["kind"] == "ferminet":
+        feats = molecule_features(xl, atoms, rescale=False)
+        h = ferminet_backbone(params, feats, nspins)
+    else:
+        feats = molecule_features(xl, atoms, rescale=True)
+        h = psiformer_backbone(params, feats, nspins, model["num_heads"])
+    phi = orbitals(params, h, nspins, K)
+    env = envelope(params, feats["r_ae"], nspins, K, model.get("envelope", "abs_isotropic"))
+    A = phi * env if env is not None else phi
+    sign, lp, _, _ = logdet_combine(A)
+    if model["kind"] == "psiformer" and model.get("jastrow", True):
+        _, r_ee = pair_within(xl)
+        lp = lp + jastrow(r_ee, nspins, float(params["jastrow"][0]), float(params["jastrow"][1]))
+    J = lp.J.reshape(B, 3 * N) if with_laplacian else None
+    return sign, Lap(lp.v, J, lp.lap)
+
+
+def orbital_matrices(model, params, x, atoms):
+    """The (B, K, N, N) orbital matrices (``orbitals()`` export), values only."""
+    xl = Lap.coords(np.asarray(x), with_tangents=False)
+    nspins = tuple(model["nspins"])
+    K = model["ndets"]
+    if model["kind"] == "ferminet":
+        feats = molecule_features(xl, atoms, rescale=False)
+        h = ferminet_backbone(params, feats, nspins)
+    else:
+        feats = molecule_features(xl, atoms, rescale=True)
+        h = psiformer_backbone(params, feats, nspins, model["num_heads"])
+    phi = orbitals(params, h, nspins, K)
+    env = envelope(params, feats["r_ae"], nspins, K, model.get("envelope", "abs_isotropic"))
+    return (phi * env).v if env is not None else phi.v
