@@ -1,0 +1,209 @@
+#!/usr/bin/env python
+"""Walker-steps/s of the per-walker VMC hot path on one MI355X.
+
+One *walker-step* = one all-electron Metropolis move of a walker (Gaussian
+proposal, log|psi| at the proposal, accept/reject) followed by its local energy
+E_L = -1/2 (lap log|psi| + |grad log|psi||^2) + V at the post-move position,
+plus the FP64 mean / variance of E_L over the batch -- the reference's
+EvaluationWorkStage.compute_step with MCMCSampler(steps=1).  A timed *step*
+is one walker-step of the whole batch; throughput = B * steps / wall.
+
+Default workload: N2-like diatomic, 14 electrons (7/7), FermiNet 4 layers
+256/32, 16 determinants, 4096 walkers, proposal width 0.02, seeded random
+parameters (synthetic data; no checkpoint exists for this system).
+
+Prints ONE JSON line on stdout; diagnostics go to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_path(spec, system, B, width, params, x0, precision, steps, warmup):
+    import torch
+
+    from jaqmc_amd import _lib
+    from jaqmc_amd.device import DeviceModel, reduce_mean_var
+
+    dm = DeviceModel(spec, system.atoms, system.charges, params, precision=precision)
+    x = torch.from_numpy(x0).cuda()
+    lp, _ = dm.logpsi(x)
+    logp = 2.0 * lp
+    acc = torch.zeros(B, dtype=torch.int32, device=x.device)
+    ctr = [0]
+
+    def step():
+        dm.mh_steps(x, logp, 1, width, seed=1234, ctr=ctr[0], n_accept=acc)
+        ctr[0] += 1
+        out = dm.local_energy(x)
+        mv = reduce_mean_var(out["e_loc"])
+        return out, mv
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out, mv = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.profile_enable(False)
+    prof = {k: _lib.profile_read(k) for k in ("gemm_layer_lap", "det_lap", "gemm_orb_lap", "pair_lap", "forward",
+                                               "local_energy")}
+    wall = t1 - t0
+    mv = mv.cpu().numpy()
+    res = dict(wall=wall, ms_per_step=1e3 * wall / steps, value=B * steps / wall, prof=prof,
+               mean=float(mv[0]), var=float(mv[1]), pmove=float(acc.sum().item()) / (B * (steps + warmup)))
+    dm.close()
+    return res
+
+
+def energies_on(spec, system, params, x, precision):
+    import torch
+
+    from jaqmc_amd.device import DeviceModel
+
+    dm = DeviceModel(spec, system.atoms, system.charges, params, precision=precision)
+    out = dm.local_energy(torch.from_numpy(x).cuda())
+    e = out["e_loc"].cpu().numpy()
+    dm.close()
+    return e
+
+
+def cpu_baseline(spec, system, params, x, width):
+    """Oracle (numpy FP32 restatement) walker-steps/s on a bounded sample."""
+    from oracle import hamiltonian as ham
+    from oracle import wavefunction as owf
+
+    m = spec.as_oracle()
+    rng = np.random.default_rng(7)
+    t0 = time.perf_counter()
+    _, lp = owf.logpsi(m, params, x, system.atoms, with_laplacian=False)
+    t_init = time.perf_counter() - t0
+    normals = rng.standard_normal(x.shape).astype(np.float32)
+    u = rng.uniform(size=x.shape[0]).astype(np.float32)
+    t0 = time.perf_counter()
+    xn, _, _, _, _ = ham.mh_step(m, params, x, 2 * lp.v, system.atoms, width, normals, u)
+    ham.local_energy(m, params, xn, system.atoms, system.charges)
+    t = time.perf_counter() - t0
+    del t_init
+    return x.shape[0] / t, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="n2", choices=["li", "n2", "ring42", "h60"])
+    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "bf16"])
+    ap.add_argument("--cpu-sample", type=int, default=256, help="walkers in the CPU-baseline sample")
+    ap.add_argument("--err-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); walkers shard with no exchange but "
+              "a single one-GPU figure is what this build reports.", file=sys.stderr)
+        sys.exit(2)
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import torch
+
+    from jaqmc_amd import params as P
+    from jaqmc_amd import roofline as RF
+    from jaqmc_amd import systems as S
+
+    system, spec, B, S_steps, width = S.workload(args.workload)
+    params = P.init_params(spec, seed=0)
+    x0 = S.init_walkers(system, B, seed=0)
+    log(f"workload {args.workload}: N={spec.n_elec} M={system.atoms.shape[0]} K={spec.ndets} B={B} "
+        f"device={torch.cuda.get_device_name(0)}")
+
+    res = {}
+    for prec in ("fp32", args.mixed):
+        res[prec] = run_path(spec, system, B, width, params, x0, prec, args.steps, args.warmup)
+        log(prec, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res[prec].items() if k != "prof"})
+        log("  prof (ms total, launches):", res[prec]["prof"])
+
+    # --- accuracy vs the FP64 oracle on a sample, and mixed vs fp32 on the full batch ---
+    from oracle import hamiltonian as ham
+
+    e32 = energies_on(spec, system, params, x0, "fp32")
+    emx = energies_on(spec, system, params, x0, args.mixed)
+    ns = min(args.err_sample, B)
+    p64 = P.cast_params(params, np.float64)
+    t0 = time.perf_counter()
+    ref = ham.local_energy(spec.as_oracle(), p64, x0[:ns].astype(np.float64), system.atoms.astype(np.float64),
+                           system.charges.astype(np.float64))["e_loc"]
+    log(f"FP64 oracle on {ns} walkers: {time.perf_counter() - t0:.1f}s")
+    rel = lambda a, b: np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+    acc_info = {
+        "fp32_vs_fp64_sample": {"n": ns, "median_rel": float(np.median(rel(e32[:ns], ref))),
+                                "p99_rel": float(np.quantile(rel(e32[:ns], ref), 0.99)),
+                                "mean_err_mHa": float(1e3 * (e32[:ns].mean() - ref.mean()))},
+        "mixed_vs_fp64_sample": {"n": ns, "median_rel": float(np.median(rel(emx[:ns], ref))),
+                                 "p99_rel": float(np.quantile(rel(emx[:ns], ref), 0.99)),
+                                 "mean_err_mHa": float(1e3 * (emx[:ns].mean() - ref.mean()))},
+        "mixed_vs_fp32_full_batch_mean_mHa": float(1e3 * (np.mean(emx) - np.mean(e32))),
+    }
+    log("accuracy:", json.dumps(acc_info))
+
+    # --- roofline of the dominant kernel class (layer GEMMs in Laplacian mode) ---
+    fl = RF.ferminet_flops(spec, system.atoms.shape[0])
+    mixed = res[args.mixed]
+    tot_ms, n = mixed["prof"]["gemm_layer_lap"]
+    lay_per_launch = fl["gemm_layer_lap"] * B / len(spec.hidden_single)
+    achieved = lay_per_launch / (tot_ms / n * 1e-3) if n else 0.0
+    peak = RF.PEAKS["bf16"]
+    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None, "kernel": "gemm_kernel (FermiNet layer, Laplacian mode)",
+            "avg_launch_ms": tot_ms / max(n, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        nc = min(args.cpu_sample, B)
+        v, t = cpu_baseline(spec, system, params, x0[:nc], width)
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": v, "unit": "walker-steps/s", "cores": cores, "kind": "port",
+               "sample": f"{nc} walkers x 1 walker-step (FP32 numpy oracle, MH move + E_L), {t:.1f}s"}
+
+    line = {
+        "metric": "walker-steps/s incl. E_L (N2 FermiNet, BF16-mixed; FP32 beside it)",
+        "value": mixed["value"],
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{args.mixed}+fp64",
+        "data": "synthetic (seeded random-init params, Gaussian walkers)",
+        "config": {"workload": f"{system.name} FermiNet 4x256/32 K={spec.ndets}", "walkers": B,
+                   "electrons": spec.n_elec, "mh_width": width},
+        "fp32_value": res["fp32"]["value"],
+        "fp32_ms_per_step": res["fp32"]["ms_per_step"],
+        "mixed_over_fp32": mixed["value"] / res["fp32"]["value"],
+        "accuracy": acc_info,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

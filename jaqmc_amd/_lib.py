@@ -20,7 +20,12 @@ PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2}
 EXPORTS = (
     "jq_model_create", "jq_model_destroy", "jq_model_workspace_bytes", "jq_logpsi", "jq_mh_steps",
     "jq_local_energy", "jq_reduce_mean_var", "jq_reduce_accepts", "jq_last_error",
+    "jq_profile_enable", "jq_profile_read",
 )
+
+# launch classes of include/jaqmc_amd_prof.h
+PROF = {"gemm_layer_lap": 0, "det_lap": 1, "gemm_orb_lap": 2, "pair_lap": 3, "forward": 4,
+        "local_energy": 5, "attn_lap": 6}
 
 
 class JqModelDesc(ctypes.Structure):
@@ -72,6 +77,8 @@ def load():
     lib.jq_reduce_mean_var.argtypes = [vp, i32, vp, vp]
     lib.jq_reduce_accepts.argtypes = [vp, i32, vp, vp]
     lib.jq_last_error.restype = ctypes.c_char_p
+    lib.jq_profile_enable.argtypes = [i32]
+    lib.jq_profile_read.argtypes = [i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype if name in (
             "jq_model_workspace_bytes", "jq_last_error") else ctypes.c_int
@@ -83,3 +90,15 @@ def check(rc: int, what: str):
     if rc != 0:
         msg = load().jq_last_error().decode(errors="replace")
         raise JqError(f"{what} failed ({rc}): {msg}")
+
+
+def profile_enable(on: bool):
+    load().jq_profile_enable(1 if on else 0)
+
+
+def profile_read(cls: str):
+    """(total device ms, launch count) of one instrumented launch class."""
+    t = ctypes.c_double()
+    n = ctypes.c_int()
+    check(load().jq_profile_read(PROF[cls], ctypes.byref(t), ctypes.byref(n)), "jq_profile_read")
+    return t.value, n.value

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "model.h"
+#include "prof.h"
 
 int gemm_unit_rows(int R);
 size_t det_smem_bytes(int N, int* Np_out, int* cpi_out);
@@ -34,7 +35,7 @@ void put_t(std::vector<float>& dst, int kpad, const float* W, int ncols, int r0,
     for (int n = 0; n < ncols; ++n) dst[(size_t)n * kpad + c0 + r] = W[(size_t)(r0 + r) * ncols + n];
 }
 
-constexpr size_t kDefaultBudget = 24ull << 30;  // bytes of walker-chunk workspace
+constexpr size_t kDefaultBudget = 16ull << 30;  // bytes of walker-chunk workspace
 
 }  // namespace
 
@@ -260,9 +261,13 @@ int jq_model::forward_ferminet(const float* elec, int Bc, bool lap, hipStream_t 
     gm.Bc = Bc;
     gm.n_up = n_up;
     gm.n_dn = n_dn;
-    rc = launch_gemm(gm, prec, xbf, s);
+    {
+      ProfScope ps(lap ? PROF_GEMM_LAYER_LAP : -1, s);
+      rc = launch_gemm(gm, prec, xbf, s);
+    }
     if (rc) return jq_fail(JQ_EINVAL, "layer GEMM launch rejected");
     if (ly.has_pair) {
+      ProfScope ps(lap ? PROF_PAIR_LAP : -1, s);
       rc = launch_pair_layer(fd, H2[hc], ly.f2, H2[hc ^ 1], ly.d2, ly.W2, ly.b2, ly.res2, s);
       hc ^= 1;
     }
@@ -292,11 +297,13 @@ int jq_model::orbitals_and_dets(const float* elec, const void* Xlast, int Bc, bo
     go.ncols = K * N;
     go.R = 1;
     go.act = ACT_NONE;
+    ProfScope ps(lap ? PROF_GEMM_ORB_LAP : -1, s);
     if (launch_gemm(go, prec, false, s)) return jq_fail(JQ_EINVAL, "orbital GEMM launch rejected");
   }
   const float* pis[2] = {pi[0], pi[1]};
   const float* sgs[2] = {sigma[0], sigma[1]};
   launch_envelope(fd, elec, atoms, pis, sgs, d.envelope, env, s);
+  ProfScope ps(lap ? PROF_DET_LAP : -1, s);
   if (launch_det(fd, Phi, ldphi, env, det, s)) return jq_fail(JQ_EINVAL, "determinant launch rejected");
   return 0;
 }
@@ -304,6 +311,7 @@ int jq_model::orbitals_and_dets(const float* elec, const void* Xlast, int Bc, bo
 int jq_model::forward(const float* elec, int b0, int Bc, bool lap, float* logpsi, float* sign, double* e_kin,
                       double* e_pot, double* e_loc, float* grad, hipStream_t s) {
   const float* ec = elec + (long)b0 * T;
+  ProfScope ps(lap ? PROF_LOCAL_ENERGY : PROF_FORWARD, s);
   int rc = kind == JQ_KIND_FERMINET ? forward_ferminet(ec, Bc, lap, s) : forward_psiformer(ec, Bc, lap, s);
   if (rc) return rc;
   CombineArgs ca{};

@@ -1,0 +1,62 @@
+"""Algorithmic work per walker for each launch class (the canonical,
+structure-aware counts used for roofline fractions).
+
+C = 3N + 2 tangent columns (value, Laplacian, 3N Jacobian rows).  The
+one-electron features of layer 0 are Local1 (5 non-zero rows per electron),
+the two-electron stream is Local2 (8 rows per pair).  Padding rows and columns
+that the kernels compute are overhead, not algorithmic work.
+"""
+
+from __future__ import annotations
+
+from .params import ModelSpec
+
+PEAKS = {  # MI355X published dense peaks
+    "fp32": 157.3e12,
+    "bf16": 2.5e15,
+    "bf16x3": 2.5e15,
+    "fp64": 78.6e12,
+    "hbm": 8.0e12,
+}
+
+
+def ferminet_flops(spec: ModelSpec, n_atoms: int):
+    """Per-walker FLOPs of the Laplacian pass, by launch class."""
+    N = spec.n_elec
+    C = 3 * N + 2
+    nch = spec.n_channels
+    K = spec.ndets
+    ins, _ = spec.ferminet_dims()
+    f1, f2 = 4 * n_atoms, 4
+    layer = spin = pair = 0.0
+    L = len(spec.hidden_single)
+    for l, (d1, d2) in enumerate(zip(spec.hidden_single, spec.hidden_double)):
+        ch = 5 if l == 0 else C
+        layer += 2.0 * N * f1 * d1 * ch + 2.0 * N * nch * f2 * d1 * C
+        spin += 2.0 * nch * f1 * d1 * C
+        if l < L - 1:
+            pair += 2.0 * N * N * f2 * d2 * 8
+            f2 = d2
+        f1 = d1
+    orb = 2.0 * N * f1 * K * N * C
+    det = K * (6.0 * N ** 4 + (2.0 / 3.0) * N ** 3 + 2.0 * N ** 3)
+    return {"gemm_layer_lap": layer, "gemm_spin_lap": spin, "pair_lap": pair, "gemm_orb_lap": orb,
+            "det_lap": det}
+
+
+def ferminet_forward_flops(spec: ModelSpec, n_atoms: int):
+    """Per-walker FLOPs of one log|psi| evaluation (C = 1)."""
+    N = spec.n_elec
+    nch = spec.n_channels
+    K = spec.ndets
+    f1, f2 = 4 * n_atoms, 4
+    tot = 0.0
+    L = len(spec.hidden_single)
+    for l, (d1, d2) in enumerate(zip(spec.hidden_single, spec.hidden_double)):
+        tot += 2.0 * N * (f1 + nch * f2) * d1 + 2.0 * nch * f1 * d1
+        if l < L - 1:
+            tot += 2.0 * N * N * f2 * d2
+            f2 = d2
+        f1 = d1
+    tot += 2.0 * N * f1 * K * N
+    return {"mfma": tot, "fp64": K * (2.0 / 3.0) * N ** 3}

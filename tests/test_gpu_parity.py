@@ -1,0 +1,208 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Tolerances (FP32 path vs the FP64 oracle on FP32 parameters and FP32 walker
+positions):
+  log|psi|  : |d| <= 1e-5 |log psi| + 1e-4
+  E_L       : median relative error <= 2e-5, max <= 2e-3 (relative to max(1, |E_L|));
+              walkers near a node amplify FP32 rounding of the orbital tangents
+  grad      : |d| <= 2e-4 max(1, |grad|) per component (median), 5e-3 (max)
+Metropolis decisions with identical proposals and uniforms agree except where
+|ratio - log u| < 1e-4 (the FP32 log-ratio error; the reference's FP32 JAX
+path has the same limit).
+"""
+
+import numpy as np
+import pytest
+
+from jaqmc_amd import params as P
+from jaqmc_amd import systems as S
+from oracle import hamiltonian as ham
+from oracle import wavefunction as owf
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "li": ("li", {}),
+    "n2_small": ("n2", dict(hidden_single=[32, 32, 32], hidden_double=[8, 16, 16], ndets=3)),
+    "n2": ("n2", {}),
+    "h60": ("h60", {}),
+    "n2_iso": ("n2", dict(hidden_single=[64, 64], hidden_double=[16, 16], ndets=2, envelope="isotropic")),
+}
+
+_cache = {}
+
+
+def _setup(case, B):
+    key = (case, B)
+    if key in _cache:
+        return _cache[key]
+    name, kw = CASES[case]
+    sysm, spec, *_ = S.workload(name)
+    for k, v in kw.items():
+        setattr(spec, k, v)
+    p = P.init_params(spec, seed=11, bias_scale=0.1, env_jitter=0.2)
+    x = S.init_walkers(sysm, B, seed=12)
+    p64 = P.cast_params(p, np.float64)
+    ref = ham.local_energy(spec.as_oracle(), p64, x.astype(np.float64), sysm.atoms.astype(np.float64),
+                           sysm.charges.astype(np.float64))
+    _cache[key] = (sysm, spec, p, x, ref)
+    return _cache[key]
+
+
+def _model(sysm, spec, p, prec="fp32", chunk=0):
+    from jaqmc_amd.device import DeviceModel
+
+    return DeviceModel(spec, sysm.atoms, sysm.charges, p, precision=prec, chunk_walkers=chunk)
+
+
+@pytest.mark.parametrize("case,B", [("li", 16), ("n2_small", 16), ("n2", 12), ("h60", 4), ("n2_iso", 8)])
+def test_logpsi_and_local_energy_fp32(cuda, case, B):
+    import torch
+
+    sysm, spec, p, x, ref = _setup(case, B)
+    dm = _model(sysm, spec, p, chunk=5)
+    xe = torch.from_numpy(x).to(cuda)
+    lp, sg = dm.logpsi(xe)
+    out = dm.local_energy(xe, with_grad=True)
+    torch.cuda.synchronize()
+    lp = lp.cpu().numpy()
+    assert np.all(np.abs(lp - ref["logpsi"]) <= 1e-5 * np.abs(ref["logpsi"]) + 1e-4)
+    np.testing.assert_array_equal(sg.cpu().numpy(), ref["sign"])
+    np.testing.assert_allclose(out["logpsi"].cpu().numpy(), lp, rtol=0, atol=1e-5 * np.abs(lp).max())
+    np.testing.assert_allclose(out["e_pot"].cpu().numpy(), ref["e_pot"], rtol=1e-6)
+    rel = np.abs(out["e_loc"].cpu().numpy() - ref["e_loc"]) / np.maximum(1.0, np.abs(ref["e_loc"]))
+    assert np.median(rel) <= 2e-5, rel
+    assert rel.max() <= 2e-3, rel
+    ek = out["e_kin"].cpu().numpy()
+    np.testing.assert_allclose(ek + out["e_pot"].cpu().numpy(), out["e_loc"].cpu().numpy(), rtol=1e-12)
+    g = out["grad"].cpu().numpy()
+    gd = np.abs(g - ref["grad"]) / np.maximum(1.0, np.abs(ref["grad"]))
+    assert np.median(gd) <= 2e-4 and gd.max() <= 5e-3
+    dm.close()
+
+
+def test_chunking_is_bit_exact(cuda):
+    import torch
+
+    sysm, spec, p, x, _ = _setup("n2_small", 16)
+    xe = torch.from_numpy(x).to(cuda)
+    a = _model(sysm, spec, p, chunk=3)
+    b = _model(sysm, spec, p, chunk=64)
+    oa, ob = a.local_energy(xe), b.local_energy(xe)
+    la, lb = a.logpsi(xe)[0], b.logpsi(xe)[0]
+    torch.cuda.synchronize()
+    assert torch.equal(oa["e_loc"], ob["e_loc"]) and torch.equal(la, lb)
+
+
+@pytest.mark.parametrize("case", ["li", "n2_small", "n2"])
+def test_metropolis_decisions_match_oracle(cuda, case):
+    import torch
+
+    sysm, spec, p, x, ref = _setup(case, 16)
+    B, N = x.shape[:2]
+    dm = _model(sysm, spec, p)
+    rng = np.random.default_rng(21)
+    steps, width = 4, 0.05
+    normals = rng.standard_normal((steps, B, N, 3)).astype(np.float32)
+    uniforms = rng.uniform(size=(steps, B)).astype(np.float32)
+    xe = torch.from_numpy(x).to(cuda)
+    logp = 2 * dm.logpsi(xe)[0]
+    acc = dm.mh_steps(xe, logp, steps, width, normals=torch.from_numpy(normals).to(cuda),
+                      uniforms=torch.from_numpy(uniforms).to(cuda))
+    torch.cuda.synchronize()
+    # oracle, step by step from the GPU's own starting state of each step
+    xo = x.astype(np.float64)
+    lpo = 2 * ref["logpsi"]
+    p64 = P.cast_params(p, np.float64)
+    nacc = np.zeros(B, int)
+    ambiguous = np.zeros(B, bool)
+    for s in range(steps):
+        xo, lpo, a, ratio, logu = ham.mh_step(spec.as_oracle(), p64, xo, lpo, sysm.atoms.astype(np.float64), width,
+                                              normals[s].astype(np.float64), uniforms[s].astype(np.float64))
+        ambiguous |= np.abs(ratio - logu) < 1e-4
+        nacc += a
+    ok = ~ambiguous
+    np.testing.assert_array_equal(acc.cpu().numpy()[ok], nacc[ok])
+    np.testing.assert_allclose(xe.cpu().numpy()[ok], xo[ok], atol=1e-5)
+    np.testing.assert_allclose(logp.cpu().numpy()[ok], lpo[ok], atol=2e-4 * np.abs(lpo).max())
+
+
+def test_philox_sampler_is_keyed_by_walker(cuda):
+    import torch
+
+    sysm, spec, p, x, _ = _setup("li", 16)
+    dm = _model(sysm, spec, p)
+    xa = torch.from_numpy(x).to(cuda)
+    xb = xa[:5].clone()
+    la = 2 * dm.logpsi(xa)[0]
+    lb = la[:5].clone()
+    aa = dm.mh_steps(xa, la, 7, 0.1, seed=99, ctr=3)
+    ab = dm.mh_steps(xb, lb, 7, 0.1, seed=99, ctr=3)
+    torch.cuda.synchronize()
+    assert torch.equal(xa[:5], xb) and torch.equal(aa[:5], ab)
+    # a different seed gives a different trajectory
+    xd = torch.from_numpy(x).to(cuda)[:5].clone()
+    ld = la[:5].clone() * 0 + 2 * dm.logpsi(xd)[0]
+    dm.mh_steps(xd, ld, 7, 0.1, seed=100, ctr=3)
+    torch.cuda.synchronize()
+    assert not torch.equal(xd, xb)
+
+
+def test_reductions_match_numpy(cuda):
+    import torch
+
+    from jaqmc_amd.device import reduce_accepts, reduce_mean_var
+
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(10007) * 5 - 3
+    v[[5, 77]] = np.nan
+    v[100] = np.inf
+    out = reduce_mean_var(torch.from_numpy(v).to(cuda)).cpu().numpy()
+    ok = v[np.isfinite(v)]
+    assert out[2] == ok.size
+    assert out[0] == pytest.approx(ok.mean(), rel=1e-12)
+    assert out[1] == pytest.approx(ok.var(), rel=1e-9)
+    a = rng.integers(0, 20, 5000).astype(np.int32)
+    assert reduce_accepts(torch.from_numpy(a).to(cuda)).item() == a.sum()
+
+
+def test_full_size_n2_batch_properties(cuda):
+    """C2 at its full size (4096 walkers): finite energies, swap antisymmetry
+    on the GPU, and the batch mean consistent with the FP64 oracle on a sample."""
+    import torch
+
+    sysm, spec, B, _, _ = S.workload("n2")
+    p = P.init_params(spec, seed=0)
+    x = S.init_walkers(sysm, B, seed=0)
+    dm = _model(sysm, spec, p)
+    xe = torch.from_numpy(x).to(cuda)
+    out = dm.local_energy(xe)
+    lp, sg = dm.logpsi(xe)
+    xs = xe.clone()
+    xs[:, [0, 1]] = xs[:, [1, 0]]
+    lps, sgs = dm.logpsi(xs)
+    torch.cuda.synchronize()
+    e = out["e_loc"].cpu().numpy()
+    assert np.isfinite(e).mean() > 0.999
+    np.testing.assert_allclose(lps.cpu().numpy(), lp.cpu().numpy(), rtol=1e-5, atol=1e-5)
+    assert torch.all(sgs == -sg)
+    ref = ham.local_energy(spec.as_oracle(), P.cast_params(p, np.float64), x[:32].astype(np.float64),
+                           sysm.atoms.astype(np.float64), sysm.charges.astype(np.float64))["e_loc"]
+    rel = np.abs(e[:32] - ref) / np.maximum(1, np.abs(ref))
+    assert np.median(rel) < 2e-5
+
+
+def test_mixed_precision_modes_track_fp32(cuda):
+    """bf16x3 (split-BF16 MFMA) keeps log|psi| within 5e-3 and the median E_L
+    within 1e-3 relative of the FP64 oracle on N2."""
+    import torch
+
+    sysm, spec, p, x, ref = _setup("n2", 12)
+    dm = _model(sysm, spec, p, prec="bf16x3")
+    xe = torch.from_numpy(x).to(cuda)
+    lp = dm.logpsi(xe)[0]
+    out = dm.local_energy(xe)
+    torch.cuda.synchronize()
+    assert np.abs(lp.cpu().numpy() - ref["logpsi"]).max() < 5e-3
+    rel = np.abs(out["e_loc"].cpu().numpy() - ref["e_loc"]) / np.maximum(1.0, np.abs(ref["e_loc"]))
+    assert np.median(rel) < 1e-3
