@@ -108,7 +108,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="n2", choices=["li", "n2", "ring42", "h60"])
-    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "bf16"])
+    ap.add_argument("--mixed", default="bf16x6", choices=["bf16x6", "bf16x3", "bf16"])
     ap.add_argument("--cpu-sample", type=int, default=256, help="walkers in the CPU-baseline sample")
     ap.add_argument("--err-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")

@@ -40,9 +40,13 @@ extern "C" {
 
 /* Arithmetic of the layer / tangent GEMMs.  Determinants (LU, inverse,
  * Laplacian traces) and walker reductions are FP64 in every mode. */
-#define JQ_PREC_FP32 0   /* FP32 MFMA, FP32 activations                        */
-#define JQ_PREC_BF16 1   /* BF16 MFMA operands + FP32 accumulate, BF16 storage */
-#define JQ_PREC_BF16X3 2 /* split-BF16 (hi*hi + hi*lo + lo*hi) MFMA, FP32 storage */
+#define JQ_PREC_FP32 0   /* FP32 MFMA                                               */
+#define JQ_PREC_BF16 1   /* BF16 operands (one term), FP32 accumulate              */
+#define JQ_PREC_BF16X3 2 /* two-term split BF16 (hh + hl + lh), FP32 accumulate    */
+#define JQ_PREC_BF16X6 3 /* three-term split BF16 (6 products), FP32 accumulate:
+                            FP32-level accuracy on BF16 matrix cores              */
+/* Activations are FP32 in HBM in every mode; operands are split into BF16
+ * terms when staged into LDS. */
 
 #define JQ_MAX_LAYERS 8
 

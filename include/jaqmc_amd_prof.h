@@ -26,6 +26,12 @@ int jq_profile_enable(int on);
  * (synchronises on the recorded events). */
 int jq_profile_read(int cls, double* total_ms, int* count);
 
+/* Time `reps` launches of the tangent GEMM kernel on synthetic zero buffers:
+ * Y (rows x ncols) = X (rows x K) W^T, R tangent rows per unit (1: plain rows),
+ * act 0/1 (none/tanh rule), prec JQ_PREC_*, variant 0 (default tiling) or
+ * 10*units_per_tile + col_blocks_per_wave.  Average milliseconds -> *ms. */
+int jq_bench_gemm(int prec, int R, int rows, int K, int ncols, int act, int variant, int reps, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

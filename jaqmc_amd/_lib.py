@@ -15,7 +15,7 @@ MAX_LAYERS = 8
 
 KIND = {"ferminet": 0, "psiformer": 1}
 ENVELOPE = {"isotropic": 0, "abs_isotropic": 1, "null": 2}
-PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2}
+PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2, "bf16x6": 3}
 
 EXPORTS = (
     "jq_model_create", "jq_model_destroy", "jq_model_workspace_bytes", "jq_logpsi", "jq_mh_steps",

@@ -21,34 +21,6 @@ int jq_fail(int code, const std::string& msg) {
 
 int gemm_unit_rows(int R);
 
-namespace {
-
-// ---- host-side weight packing ----------------------------------------------
-std::vector<uint16_t> to_bf16_bits(const std::vector<float>& v, std::vector<uint16_t>* lo) {
-  std::vector<uint16_t> hi(v.size());
-  if (lo) lo->resize(v.size());
-  for (size_t i = 0; i < v.size(); ++i) {
-    // round to nearest even
-    uint32_t b;
-    std::memcpy(&b, &v[i], 4);
-    uint32_t r = b + 0x7FFFu + ((b >> 16) & 1u);
-    if ((b & 0x7F800000u) == 0x7F800000u) r = b;  // inf/nan passthrough
-    hi[i] = (uint16_t)(r >> 16);
-    if (lo) {
-      uint32_t hb = (uint32_t)hi[i] << 16;
-      float hf;
-      std::memcpy(&hf, &hb, 4);
-      float rem = v[i] - hf;
-      uint32_t c;
-      std::memcpy(&c, &rem, 4);
-      uint32_t rr = c + 0x7FFFu + ((c >> 16) & 1u);
-      (*lo)[i] = (uint16_t)(rr >> 16);
-    }
-  }
-  return hi;
-}
-
-}  // namespace
 
 int jq_model::upload(const void* host, size_t bytes, void** dev) {
   if (hipMalloc(dev, bytes ? bytes : 16) != hipSuccess) return jq_fail(JQ_ENOMEM, "hipMalloc failed for weights");
@@ -59,20 +31,39 @@ int jq_model::upload(const void* host, size_t bytes, void** dev) {
   return 0;
 }
 
-// Upload a GEMM weight stored transposed (ncols x Kpad, zero padded), in the
-// element format of the model's precision.
+namespace {
+uint16_t bf16_rne(float f) {
+  uint32_t b;
+  std::memcpy(&b, &f, 4);
+  if ((b & 0x7F800000u) == 0x7F800000u) return (uint16_t)(b >> 16);
+  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+float bf16_to_f(uint16_t h) {
+  const uint32_t b = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+}  // namespace
+
+// Upload a GEMM weight stored transposed (ncols x Kpad, zero padded): FP32 for
+// PREC_F32, otherwise the NS-term BF16 split (x = t0 + t1 + ..., each term the
+// round-to-nearest-even BF16 of the remaining residual) as NS planes.
 int jq_model::upload_wt(const std::vector<float>& wt, void** hi, void** lo) {
-  if (prec == JQ_PREC_FP32) {
-    *lo = nullptr;
-    return upload(wt.data(), wt.size() * 4, hi);
-  }
-  std::vector<uint16_t> l;
-  std::vector<uint16_t> h = to_bf16_bits(wt, prec == JQ_PREC_BF16X3 ? &l : nullptr);
-  int rc = upload(h.data(), h.size() * 2, hi);
-  if (rc) return rc;
-  if (prec == JQ_PREC_BF16X3) return upload(l.data(), l.size() * 2, lo);
   *lo = nullptr;
-  return 0;
+  *hi = nullptr;
+  if (prec == JQ_PREC_FP32) return upload(wt.data(), wt.size() * 4, hi);
+  const int ns = prec == JQ_PREC_BF16X6 ? 3 : (prec == JQ_PREC_BF16X3 ? 2 : 1);
+  std::vector<uint16_t> planes(wt.size() * ns);
+  for (size_t i = 0; i < wt.size(); ++i) {
+    float r = wt[i];
+    for (int s = 0; s < ns; ++s) {
+      const uint16_t h = bf16_rne(r);
+      planes[s * wt.size() + i] = h;
+      r -= bf16_to_f(h);
+    }
+  }
+  return upload(planes.data(), planes.size() * 2, lo);
 }
 
 void* jq_model::alloc(size_t bytes) {

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
     const int i = idx / Np, o = idx % Np;
     double a;
     if (i < N && o < N) {
-      const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+      const int u = unit_idx(d, w, i);
       const double phi = Phi[(long)u * Rp * ldphi + k * N + o];
       a = phi * (double)env_ptr(d, env, w, i, k, o)[0];
     } else {
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
   double part = 0.0;
   for (int idx = tid; idx < N * N; idx += DET_THREADS) {
     const int i = idx / N, o = idx % N;
-    const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+    const int u = unit_idx(d, w, i);
     const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
     const float* e = env_ptr(d, env, w, i, k, o);
     double L = (double)ph[ROW_LAP * ldphi] * e[0] + (double)ph[0] * e[1];
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
       double j = 0.0;
       if (i < N && o < N) {
         const int c = c0 + cc;
-        const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+        const int u = unit_idx(d, w, i);
         const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
         const float* e = env_ptr(d, env, w, i, k, o);
         j = (double)ph[(ROW_J0 + c) * ldphi] * e[0];
@@ -249,6 +249,195 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
     o_[2] = trAL - (red[0] + red[1] + red[2] + red[3]);
   }
   for (int c = tid; c < T; c += DET_THREADS) o_[3 + c] = red[64 + c];
+}
+
+// ---------------------------------------------------------------------------
+// Small systems (N <= 16): one wavefront per determinant, four per workgroup,
+// no workgroup barriers.  The augmented [A | I] (16 x 32 FP64) is reduced by
+// Gauss-Jordan with partial pivoting in the wave's LDS slice; then lane
+// (a = lane/4, jb = lane%4) keeps inverse row a in registers and forms
+// B_c[a][4jb..4jb+3] = sum_i Inv[a][i] J_c[i][.] for each coordinate c, with
+// tr(B_c) from the diagonal lanes and tr(B_c^2) from a transposed read of the
+// wave's B_c tile.  The next J_c is prefetched from HBM while B_c is formed.
+// ---------------------------------------------------------------------------
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int SM_NP = 16;
+struct SmallDetSmem {
+  double M[SM_NP * 2 * SM_NP];  // augmented matrix, later Inv in columns 16..31
+  double J[SM_NP * SM_NP];
+  double Bt[SM_NP * SM_NP];
+  double grad[3 * SM_NP];
+};
+
+__global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+                                                        const float* __restrict__ env, double* __restrict__ out,
+                                                        int lapmode) {
+  __shared__ SmallDetSmem smem[4];
+  const int N = d.N, T = 3 * N;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long det_id = (long)blockIdx.x * 4 + wave;
+  if (det_id >= (long)d.Bc * d.K) return;  // whole wave exits together
+  const int w = (int)(det_id / d.K), k = (int)(det_id % d.K);
+  SmallDetSmem& S = smem[wave];
+  const int Rp = d.Rp;
+
+  // per-lane elements e = lane + 64 t, t = 0..3: (i, o) = (e / 16, e % 16)
+  float phi0[4], e0[4], ej[4][3];
+  long base[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int e = lane + 64 * t, i = e >> 4, o = e & 15;
+    const bool in = i < N && o < N;
+    base[t] = -1;
+    phi0[t] = 0.f;
+    e0[t] = 0.f;
+    ej[t][0] = ej[t][1] = ej[t][2] = 0.f;
+    double a = (i == o) ? 1.0 : 0.0;
+    if (in) {
+      const int u = unit_idx(d, w, i);
+      base[t] = (long)u * Rp * ldphi + k * N + o;
+      const float* ep = env + ((((long)w * N + i) * d.K + k) * N + o) * 5;
+      phi0[t] = Phi[base[t]];
+      e0[t] = ep[0];
+      if (lapmode) {
+        ej[t][0] = ep[2];
+        ej[t][1] = ep[3];
+        ej[t][2] = ep[4];
+      }
+      a = (double)phi0[t] * (double)e0[t];
+    }
+    S.M[i * 32 + o] = a;
+    S.M[i * 32 + 16 + o] = (i == o) ? 1.0 : 0.0;
+  }
+  wave_sync();
+
+  double logdet = 0.0;
+  int sgn = 1;
+  for (int p = 0; p < SM_NP; ++p) {
+    double v = (lane >= p && lane < SM_NP) ? fabs(S.M[lane * 32 + p]) : -1.0;
+    int bi = lane;
+    for (int o = 1; o < 16; o <<= 1) {
+      const double ov = __shfl_xor(v, o);
+      const int oi = __shfl_xor(bi, o);
+      if (ov > v || (ov == v && oi < bi)) {
+        v = ov;
+        bi = oi;
+      }
+    }
+    const int q = __shfl(bi, 0);
+    if (q != p) {
+      if (lane < 32) {
+        const double t0 = S.M[p * 32 + lane];
+        S.M[p * 32 + lane] = S.M[q * 32 + lane];
+        S.M[q * 32 + lane] = t0;
+      }
+      sgn = -sgn;
+    }
+    wave_sync();
+    const double pv = S.M[p * 32 + p];
+    logdet += log(fabs(pv));
+    if (pv < 0) sgn = -sgn;
+    // factors of column p for the 8 elements this lane updates (rows 2*(lane/16).., see below)
+    double f[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int e = lane + 64 * t, i = e >> 5;
+      f[t] = S.M[i * 32 + p];
+    }
+    wave_sync();
+    if (lane < 32) S.M[p * 32 + lane] /= pv;
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int e = lane + 64 * t, i = e >> 5, j = e & 31;
+      if (i != p) S.M[i * 32 + j] -= f[t] * S.M[p * 32 + j];
+    }
+    wave_sync();
+  }
+  double* o_ = out + det_id * (3 + (lapmode ? T : 0));
+  if (!lapmode) {
+    if (lane == 0) {
+      o_[0] = logdet;
+      o_[1] = (double)sgn;
+    }
+    return;
+  }
+
+  // inverse row a in registers
+  const int a = lane >> 2, jb = (lane & 3) * 4;
+  double inv[SM_NP];
+#pragma unroll
+  for (int i = 0; i < SM_NP; ++i) inv[i] = S.M[a * 32 + 16 + i];
+  for (int c = lane; c < T; c += 64) S.grad[c] = 0.0;
+
+  // tr(A^-1 L): lane's 4 elements (i, o) pair with Inv[o][i]
+  double part = 0.0;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (base[t] < 0) continue;
+    const int e = lane + 64 * t, i = e >> 4, o = e & 15;
+    const float* ep = env + ((((long)w * N + i) * d.K + k) * N + o) * 5;
+    double L = (double)Phi[base[t] + ROW_LAP * ldphi] * e0[t] + (double)phi0[t] * ep[1];
+    for (int ax = 0; ax < 3; ++ax) L += 2.0 * (double)Phi[base[t] + (ROW_J0 + 3 * i + ax) * ldphi] * ej[t][ax];
+    part += S.M[o * 32 + 16 + i] * L;
+  }
+  const double trAL = wave_sum_d(part);
+  wave_sync();
+
+  float nxt[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) nxt[t] = base[t] >= 0 ? Phi[base[t] + ROW_J0 * ldphi] : 0.f;
+  double tsum = 0.0;
+  for (int c = 0; c < T; ++c) {
+    // J_c into LDS (product rule with the envelope: only row i = c/3 gets phi * dEnv)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int e = lane + 64 * t, i = e >> 4;
+      double j = (double)nxt[t] * e0[t];
+      if (base[t] >= 0 && c / 3 == i) j += (double)phi0[t] * ej[t][c % 3];
+      S.J[e] = j;
+    }
+    wave_sync();
+    if (c + 1 < T) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) nxt[t] = base[t] >= 0 ? Phi[base[t] + (ROW_J0 + c + 1) * ldphi] : 0.f;
+    }
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+#pragma unroll
+    for (int i = 0; i < SM_NP; ++i) {
+      const double* jr = &S.J[i * 16 + jb];
+      b0 += inv[i] * jr[0];
+      b1 += inv[i] * jr[1];
+      b2 += inv[i] * jr[2];
+      b3 += inv[i] * jr[3];
+    }
+    // B^T staging: Bt[b][a] = B[a][b]
+    S.Bt[(jb + 0) * 16 + a] = b0;
+    S.Bt[(jb + 1) * 16 + a] = b1;
+    S.Bt[(jb + 2) * 16 + a] = b2;
+    S.Bt[(jb + 3) * 16 + a] = b3;
+    if (a >= jb && a < jb + 4) {
+      const double dg = a == jb ? b0 : (a == jb + 1 ? b1 : (a == jb + 2 ? b2 : b3));
+      atomicAdd(&S.grad[c], dg);
+    }
+    wave_sync();
+    // sum_b B[a][b] B[b][a] with B[b][a] = Bt[a][b]... Bt holds B^T: Bt[x][y] = B[y][x]
+    // B[b][a] for b = jb..jb+3 is Bt[a][b]
+    const double* bt = &S.Bt[a * 16 + jb];
+    tsum += b0 * bt[0] + b1 * bt[1] + b2 * bt[2] + b3 * bt[3];
+    wave_sync();
+  }
+  tsum = wave_sum_d(tsum);
+  if (lane == 0) {
+    o_[0] = logdet;
+    o_[1] = (double)sgn;
+    o_[2] = trAL - tsum;
+  }
+  for (int c = lane; c < T; c += 64) o_[3 + c] = S.grad[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -403,6 +592,12 @@ size_t det_smem_bytes(int N, int* Np_out, int* cpi_out) {
 }
 
 int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* env, double* out, hipStream_t s) {
+  if (d.N <= SM_NP) {
+    const long ndet = (long)d.Bc * d.K;
+    hipLaunchKernelGGL(det_small_kernel, dim3((unsigned)((ndet + 3) / 4)), dim3(256), 0, s, d, Phi, ldphi, env, out,
+                       d.Rp > 1 ? 1 : 0);
+    return 0;
+  }
   int Np, cpi;
   size_t smem = det_smem_bytes(d.N, &Np, &cpi);
   if (d.Rp <= 1) {  // forward mode: no Jacobian buffers

@@ -3,13 +3,13 @@
 
 #include "common.h"
 
-enum { PREC_F32 = 0, PREC_BF16 = 1, PREC_BF16X3 = 2 };
+enum { PREC_F32 = 0, PREC_BF16 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
 enum { ACT_NONE = 0, ACT_TANH = 1 };
 
 // Y[rows, ncols] = epilogue(X[rows, K] . Wt[ncols, K]^T)
 //
-// Operands are K-contiguous ("NT"): X rows and Wt rows (one per output
-// column).  K must be a multiple of 32 and ldx, ldw multiples of 8; padding
+// Operands are K-contiguous ("NT") FP32: X rows and Wt rows (one per output
+// column).  K must be a multiple of 32 and ldx, ldw multiples of 4; padding
 // columns of X and Wt must be zero.  In Laplacian mode (R > 1) rows come in
 // units of R = Rp tangent rows (value, Laplacian, Jacobian...) and one
 // workgroup owns one whole unit, so the elementwise forward-Laplacian rule of
@@ -19,7 +19,9 @@ enum { ACT_NONE = 0, ACT_TANH = 1 };
 //   residual: out = (out + Res[row]) * res_scale
 struct GemmArgs {
   const void* X; long ldx;
-  const void* Wt; const void* Wt_lo; long ldw;  // Wt_lo: low BF16 half for PREC_BF16X3
+  const void* Wt; long ldw;        // FP32 (ncols, K) transposed weight (PREC_F32)
+  const void* Wt_lo; long wplane;  // BF16 modes: NS BF16 planes of the split weight,
+                                   // plane s at Wt_lo + s * wplane (elements)
   void* Y; long ldy;
   int rows, K, ncols;
   const float* bias;   // (ncols) or null
@@ -29,10 +31,33 @@ struct GemmArgs {
   const void* Res; long ldres;  // residual source (same element type as Y) or null
   float res_scale;
   int Bc, n_up, n_dn;  // unit -> walker map (spin-blocked units)
+  int wmaj;            // units are walker-major (unit = w*N + i) instead of spin-blocked
+  int nbatch;          // > 1: blockIdx.z batches with X += z*sxb, Y += z*syb (W shared)
+  long sxb, syb;
+  // head-split output (Psiformer QKV): column n = part*D + h*dh + d of unit
+  // (w, i), row r goes to Y[(((w*3 + part)*H + h)*N + i)*R*dh + r*dh + d]
+  int hs_H, hs_dh, hs_D, hs_N;
 };
 
-// prec selects the MFMA path and the element type of X (bf16 for PREC_BF16,
-// float otherwise); y_bf16 selects the element type of Y/Res.
+// Batched GEMM on activation operands (both FP32 in HBM, split on load for the
+// BF16 modes): for z = w*bh + h, Y_z (+)= X_z . op(W_z) where op(W) is W^T
+// (Wt given as ncols x K, "NT") or W (given as K x ncols, "NN").  Rows of W
+// beyond kvalid read as zero.  Output columns may be remapped in segments:
+// col n -> (n / ycw) * ycs + n % ycw (ycw = 0: identity).
+struct BGemmArgs {
+  const float* X; long ldx; long sxw, sxh;
+  const float* W; long ldw; long sww, swh;
+  float* Y; long ldy; long syw, syh;
+  const float* Res;  // accumulate: Y = X.op(W) + Res (same addressing as Y) or null
+  int rows, K, ncols, kvalid;
+  int nw, nh;        // batch grid
+  int nn;            // 0: NT, 1: NN
+  int ycw, ycs;
+};
+int launch_bgemm(const BGemmArgs& a, int prec, hipStream_t s);
+
+// prec selects the MFMA arithmetic (PREC_*); all operands and results are
+// FP32 in memory (y_bf16 must be false).
 int launch_gemm(const GemmArgs& a, int prec, bool y_bf16, hipStream_t s);
 
 // ---- FermiNet feature / stream kernels (ferminet.hip) ----
@@ -40,7 +65,11 @@ struct FermiDims {
   int Bc, N, n_up, n_dn, M, K;  // walkers in chunk, electrons, atoms, dets
   int Rp;                       // rows per unit (1 in forward mode)
   int nch;                      // non-empty spin channels (1 or 2)
+  int wmaj;                     // walker-major units (Psiformer) instead of spin-blocked
 };
+__host__ __device__ inline int unit_idx(const FermiDims& d, int w, int i) {
+  return d.wmaj ? w * d.N + i : unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+}
 // Layer-0 one-electron input (ae features) into X[:, :, 0:4M] and the pair
 // stream input H2 (RP2 rows per pair: 8 in Laplacian mode, 1 in forward mode).
 int launch_ferminet_features(const FermiDims& d, const float* elec, const float* atoms,
@@ -83,3 +112,20 @@ int launch_accept(float* elec, const float* prop, float* logp, const float* logp
 int launch_reduce_mean_var(const double* x, int B, double* out, hipStream_t s);
 int launch_reduce_accepts(const int32_t* x, int B, double* out, hipStream_t s);
 int launch_scale_f32(float* x, int n, float a, hipStream_t s);
+
+// ---- Psiformer kernels (psiformer.hip) ----
+// Layer-0 rescaled atom-electron features + spin column into X[:, :, 0:kin).
+int launch_psi_features(const FermiDims& d, const float* elec, const float* atoms, float* X, int kin,
+                        hipStream_t s);
+// LayerNorm forward Laplacian over the feature axis, one unit (Rp rows) per
+// workgroup: y = (x - mean) rsqrt(var + eps) * scale + bias.
+int launch_layernorm_lap(int units, int Rp, int R, int D, const float* x, float* y, const float* scale,
+                         const float* bias, float eps, hipStream_t s);
+// Softmax forward Laplacian of attention scores for every (walker, head, query).
+struct SoftmaxArgs {
+  const float* S1; const float* S2; const float* S3; int lds;  // score partials
+  float* A1; int lda1;   // [z][(i,c)][j] rows c < Rp, columns lda1
+  float* A2;             // [z][i][(j,c)] = 2 * Jacobian rows (Laplacian mode)
+  int N, Rp, R, nz;
+};
+int launch_softmax_lap(const SoftmaxArgs& a, hipStream_t s);

@@ -111,6 +111,7 @@ struct jq_model {
     fd.K = K;
     fd.Rp = lap ? Rp : 1;
     fd.nch = nch;
+    fd.wmaj = kind == JQ_KIND_PSIFORMER ? 1 : 0;
     return fd;
   }
 };

@@ -49,11 +49,11 @@ int jq_model::init(const jq_model_desc& desc, const float* packed, size_t n_floa
   M = desc.n_atoms;
   K = desc.ndets;
   prec = desc.precision;
-  xbf = prec == JQ_PREC_BF16;
+  xbf = false;
   nch = (n_up > 0 && n_dn > 0) ? 2 : 1;
   if (N < 1 || N > 63 || M < 1 || K < 1 || n_up < 0 || n_dn < 0)
     return jq_fail(JQ_EINVAL, "unsupported system size (1 <= N <= 63 electrons, >= 1 atom, >= 1 det)");
-  if (prec < 0 || prec > 2) return jq_fail(JQ_EINVAL, "bad precision");
+  if (prec < 0 || prec > 3) return jq_fail(JQ_EINVAL, "bad precision");
   if (desc.envelope < 0 || desc.envelope > 2) return jq_fail(JQ_EINVAL, "bad envelope kind");
   if (!desc.atoms || !desc.charges) return jq_fail(JQ_EINVAL, "atoms / charges missing");
   Rp = gemm_unit_rows(T + 2);
@@ -228,6 +228,7 @@ int jq_model::forward_ferminet(const float* elec, int Bc, bool lap, hipStream_t 
     ga.ldx = ly.kg;
     ga.Wt = ly.Wg;
     ga.Wt_lo = ly.Wg_lo;
+    ga.wplane = (long)ly.d1 * ly.kg;
     ga.ldw = ly.kg;
     ga.Y = G;
     ga.ldy = ly.d1;
@@ -244,6 +245,7 @@ int jq_model::forward_ferminet(const float* elec, int Bc, bool lap, hipStream_t 
     gm.ldx = ly.kmain;
     gm.Wt = ly.Wm;
     gm.Wt_lo = ly.Wm_lo;
+    gm.wplane = (long)ly.d1 * ly.kmain;
     gm.ldw = ly.kmain;
     gm.Y = Xn;
     gm.ldy = ldn;
@@ -289,6 +291,7 @@ int jq_model::orbitals_and_dets(const float* elec, const void* Xlast, int Bc, bo
     go.ldx = korb;
     go.Wt = Worb[c];
     go.Wt_lo = Worb_lo[c];
+    go.wplane = (long)K * N * korb;
     go.ldw = korb;
     go.Y = Phi + u0 * RP * ldphi;
     go.ldy = ldphi;

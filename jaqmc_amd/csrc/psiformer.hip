@@ -1,0 +1,273 @@
+// Psiformer streaming kernels in the tangent-row layout (walker-major units):
+// rescaled input features, LayerNorm and the attention softmax, each with its
+// forward-Laplacian rule.
+//
+// Attention scores and outputs are batched MFMA GEMMs (gemm.hip); this file
+// holds the parts with reductions over features or keys.
+
+#include "kernels.h"
+
+namespace {
+
+inline int grid_for(long total) {
+  long g = (total + 255) / 256;
+  if (g > 65536L * 8) g = 65536L * 8;
+  return (int)(g < 1 ? 1 : g);
+}
+
+// MoleculeFeatures(rescale=True) (wavefunction/input/atomic.py:56-63) plus the
+// spin column of PsiformerBackbone (backbone/psiformer.py:165-172):
+//   f[4I]     = g = log(1 + r_iI)
+//   f[4I+1+a] = d_a g / r                (d = x_i - R_I)
+//   f[4M]     = +1 (spin up) / -1 (spin down)
+// Derivatives wrt x_i only (Local1): with q = g / r,
+//   grad g = g' d / r,  lap g = g'' + 2 g'/r
+//   d/dx_b (d_a q) = delta_ab q + d_a q' d_b / r,  lap (d_a q) = d_a (q'' + 2 q'/r) + 2 q' d_a / r.
+__global__ void psi_features_kernel(FermiDims d, const float* __restrict__ elec, const float* __restrict__ atoms,
+                                    float* __restrict__ X, int kin) {
+  const long total = (long)d.Bc * d.N * d.Rp * kin;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int f = idx % kin;
+    const long ur = idx / kin;
+    const int r = ur % d.Rp;
+    const int u = ur / d.Rp;
+    const int w = u / d.N, i = u % d.N;
+    float out = 0.f;
+    if (f < 4 * d.M) {
+      const int I = f >> 2, e = f & 3;
+      const float* xi = elec + ((long)w * d.N + i) * 3;
+      const float dv[3] = {xi[0] - atoms[3 * I], xi[1] - atoms[3 * I + 1], xi[2] - atoms[3 * I + 2]};
+      const float rr = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+      const float g = log1pf(rr), g1 = 1.f / (1.f + rr), g2 = -g1 * g1;
+      if (e == 0) {
+        if (r == ROW_VAL)
+          out = g;
+        else if (r == ROW_LAP)
+          out = g2 + 2.f * g1 / rr;
+        else if (r - ROW_J0 < 3 * d.N && (r - ROW_J0) / 3 == i)
+          out = g1 * dv[(r - ROW_J0) % 3] / rr;
+      } else {
+        const int ax = e - 1;
+        const float q = g / rr;
+        const float q1 = g1 / rr - g / (rr * rr);
+        const float q2 = g2 / rr - 2.f * g1 / (rr * rr) + 2.f * g / (rr * rr * rr);
+        if (r == ROW_VAL)
+          out = dv[ax] * q;
+        else if (r == ROW_LAP)
+          out = dv[ax] * (q2 + 2.f * q1 / rr) + 2.f * q1 * dv[ax] / rr;
+        else if (r - ROW_J0 < 3 * d.N && (r - ROW_J0) / 3 == i) {
+          const int b = (r - ROW_J0) % 3;
+          out = (b == ax ? q : 0.f) + dv[ax] * q1 * dv[b] / rr;
+        }
+      }
+    } else if (f == 4 * d.M && r == ROW_VAL) {
+      out = i < d.n_up ? 1.f : -1.f;
+    }
+    X[ur * kin + f] = out;
+  }
+}
+
+// LayerNorm forward Laplacian (flax nn.LayerNorm, epsilon 1e-5;
+// backbone/psiformer.py:74,85).  One workgroup per unit holds the unit's
+// Rp x D tangent tile in LDS.  With d = x - mean(x) (linear), v = mean(d0^2),
+// s = (v + eps)^-1/2:
+//   J_v = 2/D sum_f d0 J_d,     lap_v = 2/D sum_f (d0 lap_d + sum_c J_d^2)
+//   J_s = s' J_v,                lap_s = s' lap_v + s'' sum_c J_v^2
+//   y = scale (d s) + bias:  J_y = scale (J_d s + d0 J_s),
+//   lap_y = scale (lap_d s + d0 lap_s + 2 sum_c J_d J_s)
+__global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, int D, const float* __restrict__ x,
+                                                            float* __restrict__ y, const float* __restrict__ scale,
+                                                            const float* __restrict__ bias, float eps) {
+  extern __shared__ float sm[];
+  float* T = sm;              // Rp x D
+  float* mu = T + Rp * D;     // Rp
+  float* dot0 = mu + Rp;      // Rp
+  float* sq = dot0 + Rp;      // Rp
+  float* Js = sq + Rp;        // Rp
+  float* scal = Js + Rp;      // 4
+  const long base = (long)blockIdx.x * Rp * D;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int idx = tid; idx < Rp * D; idx += 256) T[idx] = x[base + idx];
+  __syncthreads();
+  for (int r = wave; r < Rp; r += 4) {
+    float s = 0.f;
+    for (int f = lane; f < D; f += 64) s += T[r * D + f];
+    s = wave_sum(s);
+    if (lane == 0) mu[r] = s / D;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < Rp * D; idx += 256) T[idx] -= mu[idx / D];
+  __syncthreads();
+  for (int r = wave; r < Rp; r += 4) {
+    float s0 = 0.f, s1 = 0.f;
+    for (int f = lane; f < D; f += 64) {
+      const float dr = T[r * D + f];
+      s0 += T[f] * dr;
+      s1 += dr * dr;
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if (lane == 0) {
+      dot0[r] = s0;
+      sq[r] = s1;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const float v = dot0[0] / D;
+    float lapv = dot0[1];
+    for (int r = ROW_J0; r < R; ++r) lapv += sq[r];
+    lapv *= 2.f / D;
+    const float ve = v + eps;
+    const float s = rsqrtf(ve);
+    const float s1 = -0.5f * s / ve;
+    const float s2 = 0.75f * s / (ve * ve);
+    float jv2 = 0.f;
+    for (int r = ROW_J0; r < R; ++r) {
+      const float jv = 2.f * dot0[r] / D;
+      Js[r] = s1 * jv;
+      jv2 += jv * jv;
+    }
+    scal[0] = s;
+    scal[1] = s1 * lapv + s2 * jv2;  // lap_s
+  }
+  __syncthreads();
+  for (int r = tid; r < Rp; r += 256)
+    if (r < ROW_J0 || r >= R) Js[r] = 0.f;
+  __syncthreads();
+  const float s = scal[0], laps = scal[1];
+  for (int f = tid; f < D; f += 256) {
+    const float g = scale[f], d0 = T[f];
+    float cross = 0.f;
+    for (int r = ROW_J0; r < R; ++r) cross += T[r * D + f] * Js[r];
+    y[base + f] = g * d0 * s + bias[f];
+    if (Rp > 1) y[base + D + f] = g * (T[D + f] * s + d0 * laps + 2.f * cross);
+    for (int r = ROW_J0; r < Rp; ++r) y[base + (long)r * D + f] = r < R ? g * (T[r * D + f] * s + d0 * Js[r]) : 0.f;
+  }
+}
+
+// Softmax forward Laplacian over keys j for query i of (walker, head) z:
+//   s_j: value S1[(i,0)][j]; J_c = S1[(i,c)][j] + S2[(j,c)][i];
+//        lap = S1[(i,1)][j] + S2[(j,1)][i] + 2 S3[i][j]
+//   a = softmax(s);  mu_c = sum_j a_j J_jc;  delta_jc = J_jc - mu_c
+//   J_a_jc = a_j delta_jc
+//   lap_a_j = a_j (sum_c delta_jc^2 - sum_k a_k sum_c delta_kc^2 + lap_j - sum_k a_k lap_k)
+__global__ __launch_bounds__(256) void softmax_lap_kernel(SoftmaxArgs a) {
+  extern __shared__ float sm[];
+  const int N = a.N, Rp = a.Rp, R = a.R;
+  float* J = sm;              // Rp x N (rows c >= 2 used)
+  float* av = J + Rp * N;     // N: a_j
+  float* lp = av + N;         // N: lap_j
+  float* rj = lp + N;         // N
+  float* red = rj + N;        // 8
+  const int z = blockIdx.x / N, i = blockIdx.x % N;
+  const int tid = threadIdx.x;
+  const long zS = (long)z * N * Rp * a.lds;
+  const float* S1 = a.S1 + zS + (long)i * Rp * a.lds;
+  for (int j = tid; j < N; j += 256) {
+    av[j] = S1[j];
+    if (Rp > 1) lp[j] = S1[a.lds + j] + a.S2[zS + ((long)j * Rp + 1) * a.lds + i] +
+                        2.f * a.S3[(long)z * N * a.lds + (long)i * a.lds + j];
+  }
+  if (Rp > 1)
+    for (int idx = tid; idx < (R - ROW_J0) * N; idx += 256) {
+      const int c = ROW_J0 + idx / N, j = idx % N;
+      J[c * N + j] = S1[(long)c * a.lds + j] + a.S2[zS + ((long)j * Rp + c) * a.lds + i];
+    }
+  __syncthreads();
+  if (tid < 64) {
+    float m = -INFINITY;
+    for (int j = tid; j < N; j += 64) m = fmaxf(m, av[j]);
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    float zsum = 0.f;
+    for (int j = tid; j < N; j += 64) zsum += expf(av[j] - m);
+    zsum = wave_sum(zsum);
+    for (int j = tid; j < N; j += 64) av[j] = expf(av[j] - m) / zsum;
+  }
+  __syncthreads();
+  float* A1 = a.A1 + (long)z * N * Rp * a.lda1 + (long)i * Rp * a.lda1;
+  if (Rp == 1) {
+    for (int j = tid; j < a.lda1; j += 256) A1[j] = j < N ? av[j] : 0.f;
+    return;
+  }
+  // mu_c and delta
+  for (int c = ROW_J0 + tid; c < R; c += 256) {
+    float mu = 0.f;
+    for (int j = 0; j < N; ++j) mu += av[j] * J[c * N + j];
+    for (int j = 0; j < N; ++j) J[c * N + j] -= mu;
+  }
+  __syncthreads();
+  for (int j = tid; j < N; j += 256) {
+    float s = 0.f;
+    for (int c = ROW_J0; c < R; ++c) s += J[c * N + j] * J[c * N + j];
+    rj[j] = s;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    float nu = 0.f, lam = 0.f;
+    for (int j = tid; j < N; j += 64) {
+      nu += av[j] * rj[j];
+      lam += av[j] * lp[j];
+    }
+    nu = wave_sum(nu);
+    lam = wave_sum(lam);
+    if (tid == 0) {
+      red[0] = nu;
+      red[1] = lam;
+    }
+  }
+  __syncthreads();
+  const float nu = red[0], lam = red[1];
+  for (int idx = tid; idx < Rp * a.lda1; idx += 256) {
+    const int c = idx / a.lda1, j = idx % a.lda1;
+    float v = 0.f;
+    if (j < N) {
+      if (c == ROW_VAL)
+        v = av[j];
+      else if (c == ROW_LAP)
+        v = av[j] * (rj[j] - nu + lp[j] - lam);
+      else if (c < R)
+        v = av[j] * J[c * N + j];
+    }
+    A1[idx] = v;
+  }
+  if (a.A2) {
+    const long lda2 = ((long)N * Rp + 31) / 32 * 32;
+    float* A2 = a.A2 + (long)z * N * lda2 + (long)i * lda2;
+    for (long idx = tid; idx < lda2; idx += 256) {
+      const int j = (int)(idx / Rp), c = (int)(idx % Rp);
+      float v = 0.f;
+      if (j < N && c >= ROW_J0 && c < R) v = 2.f * av[j] * J[c * N + j];
+      A2[idx] = v;
+    }
+  }
+}
+
+}  // namespace
+
+int launch_psi_features(const FermiDims& d, const float* elec, const float* atoms, float* X, int kin, hipStream_t s) {
+  const long total = (long)d.Bc * d.N * d.Rp * kin;
+  hipLaunchKernelGGL(psi_features_kernel, grid_for(total), 256, 0, s, d, elec, atoms, X, kin);
+  return 0;
+}
+
+int launch_layernorm_lap(int units, int Rp, int R, int D, const float* x, float* y, const float* scale,
+                         const float* bias, float eps, hipStream_t s) {
+  const size_t smem = ((size_t)Rp * D + 4 * Rp + 4) * 4;
+  if (smem > 160 * 1024) return -1;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)layernorm_lap_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL(layernorm_lap_kernel, dim3(units), dim3(256), smem, s, Rp, R, D, x, y, scale, bias, eps);
+  return 0;
+}
+
+int launch_softmax_lap(const SoftmaxArgs& a, hipStream_t s) {
+  const size_t smem = ((size_t)a.Rp * a.N + 3 * a.N + 8) * 4;
+  if (smem > 160 * 1024) return -1;
+  hipLaunchKernelGGL(softmax_lap_kernel, dim3(a.nz * a.N), dim3(256), smem, s, a);
+  return 0;
+}

@@ -1,11 +1,391 @@
-// Psiformer pipeline (wavefunction/backbone/psiformer.py) -- constructed in
-// psiformer.hip once available.
-#include "model.h"
+// Psiformer pipeline (app/molecule/wavefunction/psiformer.py:93-166,
+// wavefunction/backbone/psiformer.py:29-187): weight packing and the
+// per-chunk forward / forward-Laplacian sequence.
+//
+// Per layer (pre-LN):  x += Wo . MHA(LN1(x));  x += tanh(W2 tanh(W1 LN2(x)))
+// Attention per (walker w, head h), q pre-scaled by 1/sqrt(dh) in the weights:
+//   S1[(i,c)][j] = sum_d Q[i][c][d] K[j][0][d]          (Q Jacobian/lap rows x key values)
+//   S2[(j,c)][i] = sum_d K[j][c][d] Q[i][0][d]
+//   S3[i][j]     = sum_{c>=2,d} Q[i][c][d] K[j][c][d]    (Laplacian cross term)
+//   softmax_lap -> A1[(i,c)][j] (value, Laplacian, Jacobian of a), A2[i][(j,c)] = 2 J_a
+//   O[(i,c)][d]  = sum_j A1[(i,c)][j] V[j][0][d] + [c >= 1] sum_j a[i][j] V[j][c][d]
+//   O[(i,1)][d] += sum_{j,c>=2} 2 J_a[i][c][j] J_V[j][c][d]
+// which is the product rule of laplacian/primitives/core.py:494-500 applied to
+// q.k and a.v with dense Jacobians.
 
-int jq_model::init_psiformer(const float*, size_t) {
-  return jq_fail(JQ_EINVAL, "Psiformer is not built in this library version");
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "model.h"
+#include "prof.h"
+
+size_t det_smem_bytes(int N, int* Np_out, int* cpi_out);
+
+namespace {
+struct Reader {
+  const float* p;
+  size_t n, pos = 0;
+  bool ok = true;
+  const float* take(size_t k) {
+    if (pos + k > n) {
+      ok = false;
+      return nullptr;
+    }
+    const float* r = p + pos;
+    pos += k;
+    return r;
+  }
+};
+
+// dst (ncols x kpad) at column offset c0 <- transposed rows [r0, r0+nr) of W
+// (rows x ldw_in, taking columns [cin0, cin0 + ncols)), scaled.
+void put_t(std::vector<float>& dst, int kpad, const float* W, int ldw_in, int cin0, int ncols, int r0, int nr,
+           int row_out0, float scale = 1.f) {
+  for (int r = 0; r < nr; ++r)
+    for (int n = 0; n < ncols; ++n)
+      dst[(size_t)(row_out0 + n) * kpad + r] = scale * W[(size_t)(r0 + r) * ldw_in + cin0 + n];
 }
 
-int jq_model::forward_psiformer(const float*, int, bool, hipStream_t) {
-  return jq_fail(JQ_EINVAL, "Psiformer is not built in this library version");
+constexpr size_t kBudget = 16ull << 30;
+}  // namespace
+
+int jq_model::init_psiformer(const float* packed, size_t n_floats) {
+  L = d.n_layers;
+  H = d.num_heads;
+  dh = d.heads_dim;
+  D = H * dh;
+  if (L < 1 || H < 1 || dh < 4 || dh % 32 != 0 || d.n_mlp < 0 || d.n_mlp > 3)
+    return jq_fail(JQ_EINVAL, "unsupported Psiformer shape (heads_dim must be a multiple of 32)");
+  if (Rp > 128) return jq_fail(JQ_EINVAL, "Psiformer supports N <= 42 electrons (tangent tile <= 128 rows)");
+  Reader rd{packed, n_floats};
+  const int fin = 4 * M + 1;
+  kin = round_up(fin, 32);
+  const float* Wi = rd.take((size_t)fin * D);
+  const float* bi = rd.take(D);
+  if (!rd.ok) return jq_fail(JQ_EINVAL, "parameter pack too short (Psiformer input)");
+  {
+    std::vector<float> wt((size_t)D * kin, 0.f);
+    put_t(wt, kin, Wi, D, 0, D, 0, fin, 0);
+    int rc = upload_wt(wt, &Win, &Win_lo);
+    if (!rc) rc = upload(bi, 4L * D, (void**)&bin);
+    if (rc) return rc;
+  }
+  const float qs = 1.f / std::sqrt((float)dh);
+  pl.resize(L);
+  for (int l = 0; l < L; ++l) {
+    PsiLayer& ly = pl[l];
+    const float* l1s = rd.take(D);
+    const float* l1b = rd.take(D);
+    const float* wq = rd.take((size_t)D * D);
+    const float* bq = rd.take(D);
+    const float* wk = rd.take((size_t)D * D);
+    const float* bk = rd.take(D);
+    const float* wv = rd.take((size_t)D * D);
+    const float* bv = rd.take(D);
+    const float* wo = rd.take((size_t)D * D);
+    const float* bo = rd.take(D);
+    const float* l2s = rd.take(D);
+    const float* l2b = rd.take(D);
+    ly.mdims.clear();
+    ly.mdims.push_back(D);
+    for (int j = 0; j < d.n_mlp; ++j) ly.mdims.push_back(d.mlp_hidden[j]);
+    ly.mdims.push_back(D);
+    std::vector<const float*> mw, mb;
+    for (size_t j = 0; j + 1 < ly.mdims.size(); ++j) {
+      mw.push_back(rd.take((size_t)ly.mdims[j] * ly.mdims[j + 1]));
+      mb.push_back(rd.take(ly.mdims[j + 1]));
+    }
+    if (!rd.ok) return jq_fail(JQ_EINVAL, "parameter pack too short (Psiformer layer)");
+    int rc = upload(l1s, 4L * D, (void**)&ly.ln1_s);
+    if (!rc) rc = upload(l1b, 4L * D, (void**)&ly.ln1_b);
+    if (!rc) rc = upload(l2s, 4L * D, (void**)&ly.ln2_s);
+    if (!rc) rc = upload(l2b, 4L * D, (void**)&ly.ln2_b);
+    std::vector<float> wqkv((size_t)3 * D * D, 0.f), bqkv(3 * D);
+    put_t(wqkv, D, wq, D, 0, D, 0, D, 0, qs);
+    put_t(wqkv, D, wk, D, 0, D, 0, D, D);
+    put_t(wqkv, D, wv, D, 0, D, 0, D, 2 * D);
+    for (int n = 0; n < D; ++n) {
+      bqkv[n] = qs * bq[n];
+      bqkv[D + n] = bk[n];
+      bqkv[2 * D + n] = bv[n];
+    }
+    if (!rc) rc = upload_wt(wqkv, &ly.Wqkv, &ly.Wqkv_lo);
+    if (!rc) rc = upload(bqkv.data(), 12L * D, (void**)&ly.bqkv);
+    std::vector<float> wot((size_t)D * D, 0.f);
+    put_t(wot, D, wo, D, 0, D, 0, D, 0);
+    if (!rc) rc = upload_wt(wot, &ly.Wo, &ly.Wo_lo);
+    if (!rc) rc = upload(bo, 4L * D, (void**)&ly.bo);
+    ly.Wm.resize(mw.size());
+    ly.Wm_lo.resize(mw.size());
+    ly.bm.resize(mw.size());
+    for (size_t j = 0; j < mw.size() && !rc; ++j) {
+      const int a = ly.mdims[j], b = ly.mdims[j + 1];
+      const int ka = round_up(a, 32);
+      std::vector<float> wt((size_t)b * ka, 0.f);
+      put_t(wt, ka, mw[j], b, 0, b, 0, a, 0);
+      rc = upload_wt(wt, &ly.Wm[j], &ly.Wm_lo[j]);
+      if (!rc) rc = upload(mb[j], 4L * b, (void**)&ly.bm[j]);
+    }
+    if (rc) return rc;
+  }
+  // outputs (orbital projection, envelope), then the Jastrow
+  {
+    korb = round_up(D, 32);
+    const int ncol = K * N;
+    for (int c = 0; c < nch; ++c) {
+      const float* W = rd.take((size_t)D * ncol);
+      if (!W) return jq_fail(JQ_EINVAL, "parameter pack too short (orbital)");
+      std::vector<float> wt((size_t)ncol * korb, 0.f);
+      put_t(wt, korb, W, ncol, 0, ncol, 0, D, 0);
+      int rc = upload_wt(wt, &Worb[c], &Worb_lo[c]);
+      if (rc) return rc;
+    }
+    const size_t ne = (size_t)N * M * K;
+    for (int c = 0; c < nch; ++c) {
+      const float* pp = rd.take(ne);
+      const float* sg = rd.take(ne);
+      if (!pp || !sg) return jq_fail(JQ_EINVAL, "parameter pack too short (envelope)");
+      int rc = upload(pp, ne * 4, (void**)&pi[c]);
+      if (!rc) rc = upload(sg, ne * 4, (void**)&sigma[c]);
+      if (rc) return rc;
+    }
+  }
+  const float* jas_p = rd.take(2);
+  if (!jas_p) return jq_fail(JQ_EINVAL, "parameter pack too short (Jastrow)");
+  jas[0] = jas_p[0];
+  jas[1] = jas_p[1];
+  if (rd.pos != n_floats) return jq_fail(JQ_EINVAL, "parameter pack size mismatch");
+
+  // ---- workspace ----
+  int hmax = D;
+  for (auto& ly : pl)
+    for (int v : ly.mdims) hmax = std::max(hmax, round_up(v, 32));
+  const long NpS = round_up(N, 4), Np32 = round_up(N, 32);
+  auto per_walker = [&](long R) {
+    const long u = (long)N * R;
+    const long lda2 = (long)round_up((int)(N * R), 32);
+    return (size_t)4 * (u * D + u * hmax + u * std::max<long>(hmax, kin) + 3 * u * D + u * D +
+                        2L * H * u * NpS + (long)H * N * NpS + (long)H * u * Np32 +
+                        (R > 1 ? (long)H * N * lda2 : 0) + u * round_up(K * N, 4) + (long)N * K * N * 5) +
+           (size_t)8 * K * (3 + T) + 64;
+  };
+  ldphi = round_up(K * N, 4);
+  const size_t pw_lap = per_walker(Rp), pw_fwd = per_walker(1);
+  chunk = d.chunk_walkers > 0 ? d.chunk_walkers : (int)std::max<size_t>(1, kBudget / pw_lap);
+  chunk = std::min(chunk, 1 << 16);
+  chunk_fwd = (int)std::min<size_t>((size_t)chunk * pw_lap / pw_fwd, 1 << 18);
+  auto nmax = [&](long per_lap, long per_fwd) {
+    return (size_t)std::max((long)chunk * per_lap, (long)chunk_fwd * per_fwd);
+  };
+  const long uL = (long)N * Rp, uF = N;
+  const long lda2 = round_up(N * Rp, 32);
+  P[0] = alloc(4 * nmax(uL * D, uF * D));                                       // x
+  P[1] = alloc(4 * nmax(uL * hmax, uF * hmax));                                 // T1
+  P[2] = alloc(4 * nmax(uL * std::max(hmax, kin), uF * std::max(hmax, kin)));   // T2 / X0
+  P[3] = alloc(4 * nmax(3 * uL * D, 3 * uF * D));                               // QKV (head-split)
+  P[4] = alloc(4 * nmax(uL * D, uF * D));                                       // attention output
+  P[5] = alloc(4 * nmax(2L * H * uL * NpS + (long)H * N * NpS + (long)H * uL * Np32 + (long)H * N * lda2,
+                        3L * H * uF * NpS + (long)H * uF * Np32));              // scores / softmax
+  Phi = (float*)alloc(4 * nmax(uL * ldphi, uF * ldphi));
+  env = (float*)alloc(4 * (size_t)std::max(chunk, chunk_fwd) * N * K * N * 5);
+  det = (double*)alloc(8 * nmax((long)K * (3 + T), (long)K * 3));
+  for (int i = 0; i < 6; ++i)
+    if (!P[i]) return jq_fail(JQ_ENOMEM, "workspace allocation failed");
+  if (!Phi || !env || !det) return jq_fail(JQ_ENOMEM, "workspace allocation failed");
+  return 0;
+}
+
+int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t s) {
+  const FermiDims fd = dims(Bc, lap);
+  const int RP = fd.Rp;
+  const int R = lap ? T + 2 : 1;
+  const long units = (long)Bc * N;
+  float* x = (float*)P[0];
+  float* T1 = (float*)P[1];
+  float* T2 = (float*)P[2];
+  float* QKV = (float*)P[3];
+  float* Om = (float*)P[4];
+  const int NpS = round_up(N, 4), Np32 = round_up(N, 32);
+  const long lda2 = round_up(N * RP, 32);
+  float* S1 = (float*)P[5];
+  float* S2 = S1 + (long)Bc * H * N * RP * NpS;
+  float* S3 = S2 + (long)Bc * H * N * RP * NpS;
+  float* A1 = S3 + (long)Bc * H * N * NpS;
+  float* A2 = A1 + (long)Bc * H * N * RP * Np32;
+
+  auto gemm = [&](const float* X, long ldx, void* Wt, void* Wlo, long wplane, int K_, float* Y, long ldy, int ncols,
+                  const float* bias, int act, const float* Res) {
+    GemmArgs g{};
+    g.X = X;
+    g.ldx = ldx;
+    g.Wt = Wt;
+    g.Wt_lo = Wlo;
+    g.wplane = wplane;
+    g.ldw = K_;
+    g.Y = Y;
+    g.ldy = ldy;
+    g.rows = (int)(units * RP);
+    g.K = K_;
+    g.ncols = ncols;
+    g.bias = bias;
+    g.R = RP;
+    g.act = act;
+    g.Res = Res;
+    g.ldres = ldy;
+    g.res_scale = 1.f;
+    g.Bc = Bc;
+    g.n_up = n_up;
+    g.n_dn = n_dn;
+    g.wmaj = 1;
+    return launch_gemm(g, prec, false, s);
+  };
+
+  int rc = launch_psi_features(fd, elec, atoms, T2, kin, s);
+  rc |= gemm(T2, kin, Win, Win_lo, (long)D * kin, kin, x, D, D, bin, ACT_NONE, nullptr);
+  for (int l = 0; l < L && !rc; ++l) {
+    const PsiLayer& ly = pl[l];
+    rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln1_s, ly.ln1_b, 1e-5f, s);
+    {
+      GemmArgs g{};
+      g.X = T1;
+      g.ldx = D;
+      g.Wt = ly.Wqkv;
+      g.Wt_lo = ly.Wqkv_lo;
+      g.wplane = 3L * D * D;
+      g.ldw = D;
+      g.Y = QKV;
+      g.ldy = 3 * D;
+      g.rows = (int)(units * RP);
+      g.K = D;
+      g.ncols = 3 * D;
+      g.bias = ly.bqkv;
+      g.R = RP;
+      g.act = ACT_NONE;
+      g.res_scale = 1.f;
+      g.wmaj = 1;
+      g.hs_H = H;
+      g.hs_dh = dh;
+      g.hs_D = D;
+      g.hs_N = N;
+      rc |= launch_gemm(g, prec, false, s);
+    }
+    // ---- attention ----
+    {
+      ProfScope ps(lap ? PROF_ATTN_LAP : -1, s);
+      const long blk = (long)N * RP * dh;   // one (w, part, h) block of QKV
+      const long sw = 3L * H * blk;
+      const float* Q = QKV;
+      const float* Kx = QKV + (long)H * blk;
+      const float* V = QKV + 2L * H * blk;
+      BGemmArgs b{};
+      // S1 = Q_rows . K_values
+      b = BGemmArgs{};
+      b.X = Q; b.ldx = dh; b.sxw = sw; b.sxh = blk;
+      b.W = Kx; b.ldw = (long)RP * dh; b.sww = sw; b.swh = blk;
+      b.Y = S1; b.ldy = NpS; b.syw = (long)H * N * RP * NpS; b.syh = (long)N * RP * NpS;
+      b.rows = N * RP; b.K = dh; b.ncols = N; b.kvalid = dh; b.nw = Bc; b.nh = H; b.nn = 0;
+      rc |= launch_bgemm(b, prec, s);
+      if (lap) {
+        // S2 = K_rows . Q_values
+        b.X = Kx;
+        b.W = Q;
+        b.Y = S2;
+        rc |= launch_bgemm(b, prec, s);
+        // S3 = cross term over Jacobian rows
+        BGemmArgs c3{};
+        c3.X = Q + 2 * dh; c3.ldx = (long)RP * dh; c3.sxw = sw; c3.sxh = blk;
+        c3.W = Kx + 2 * dh; c3.ldw = (long)RP * dh; c3.sww = sw; c3.swh = blk;
+        c3.Y = S3; c3.ldy = NpS; c3.syw = (long)H * N * NpS; c3.syh = (long)N * NpS;
+        c3.rows = N; c3.K = (RP - 2) * dh; c3.ncols = N; c3.kvalid = 0; c3.nw = Bc; c3.nh = H; c3.nn = 0;
+        rc |= launch_bgemm(c3, prec, s);
+      }
+      SoftmaxArgs sa{};
+      sa.S1 = S1;
+      sa.S2 = S2;
+      sa.S3 = S3;
+      sa.lds = NpS;
+      sa.A1 = A1;
+      sa.lda1 = Np32;
+      sa.A2 = lap ? A2 : nullptr;
+      sa.N = N;
+      sa.Rp = RP;
+      sa.R = R;
+      sa.nz = Bc * H;
+      rc |= launch_softmax_lap(sa, s);
+      // O = A1 . V_values   (all rows of a; NN)
+      BGemmArgs g4{};
+      g4.X = A1; g4.ldx = Np32; g4.sxw = (long)H * N * RP * Np32; g4.sxh = (long)N * RP * Np32;
+      g4.W = V; g4.ldw = (long)RP * dh; g4.sww = sw; g4.swh = blk;
+      g4.Y = Om; g4.ldy = D; g4.syw = (long)N * RP * D; g4.syh = dh;
+      g4.rows = N * RP; g4.K = Np32; g4.ncols = dh; g4.kvalid = N; g4.nw = Bc; g4.nh = H; g4.nn = 1;
+      rc |= launch_bgemm(g4, prec, s);
+      if (lap) {
+        // O[i][(c,d)] += a[i][:] . V[:, (c,d)] for the Laplacian and Jacobian rows c >= 1
+        // (the value row a.v came from the previous product)
+        BGemmArgs g5{};
+        g5.X = A1; g5.ldx = (long)RP * Np32; g5.sxw = g4.sxw; g5.sxh = g4.sxh;
+        g5.W = V + dh; g5.ldw = (long)RP * dh; g5.sww = sw; g5.swh = blk;
+        g5.Y = Om + D; g5.ldy = (long)RP * D; g5.syw = g4.syw; g5.syh = dh; g5.Res = Om + D;
+        g5.rows = N; g5.K = Np32; g5.ncols = (RP - 1) * dh; g5.kvalid = N; g5.nw = Bc; g5.nh = H; g5.nn = 1;
+        g5.ycw = dh; g5.ycs = D;
+        rc |= launch_bgemm(g5, prec, s);
+        // O[i][1][d] += sum_{j,c} 2 J_a[i][(j,c)] V[(j,c)][d]
+        BGemmArgs g6{};
+        g6.X = A2; g6.ldx = lda2; g6.sxw = (long)H * N * lda2; g6.sxh = (long)N * lda2;
+        g6.W = V; g6.ldw = dh; g6.sww = sw; g6.swh = blk;
+        g6.Y = Om + D; g6.ldy = (long)RP * D; g6.syw = g4.syw; g6.syh = dh; g6.Res = Om + D;
+        g6.rows = N; g6.K = (int)lda2; g6.ncols = dh; g6.kvalid = N * RP; g6.nw = Bc; g6.nh = H; g6.nn = 1;
+        rc |= launch_bgemm(g6, prec, s);
+      }
+    }
+    // x += Wo O + bo
+    rc |= gemm(Om, D, ly.Wo, ly.Wo_lo, (long)D * D, D, x, D, D, ly.bo, ACT_NONE, x);
+    // MLP on LN2(x), residual into x
+    rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln2_s, ly.ln2_b, 1e-5f, s);
+    float* cur = T1;
+    int curw = D;
+    const int nm = (int)ly.Wm.size();
+    for (int j = 0; j < nm && !rc; ++j) {
+      const int outw = ly.mdims[j + 1];
+      const int ka = round_up(curw, 32);
+      const bool last = j == nm - 1;
+      float* dst = last ? x : (cur == T1 ? T2 : T1);
+      rc |= gemm(cur, last ? ka : ka, ly.Wm[j], ly.Wm_lo[j], (long)outw * ka, ka, dst, last ? D : round_up(outw, 32),
+                 outw, ly.bm[j], ACT_TANH, last ? x : nullptr);
+      cur = dst;
+      curw = outw;
+    }
+  }
+  if (rc) return jq_fail(JQ_EINVAL, "Psiformer launch rejected");
+  // orbitals: per spin channel, batched over walkers (walker-major units)
+  for (int c = 0; c < nch; ++c) {
+    const int i0 = (nch == 1 || c == 0) ? 0 : n_up;
+    const int ns = nch == 1 ? N : (c == 0 ? n_up : n_dn);
+    GemmArgs go{};
+    go.X = x + (long)i0 * RP * D;
+    go.ldx = D;
+    go.Wt = Worb[c];
+    go.Wt_lo = Worb_lo[c];
+    go.wplane = (long)K * N * korb;
+    go.ldw = korb;
+    go.Y = Phi + (long)i0 * RP * ldphi;
+    go.ldy = ldphi;
+    go.rows = ns * RP;
+    go.K = korb;
+    go.ncols = K * N;
+    go.R = 1;
+    go.act = ACT_NONE;
+    go.nbatch = Bc;
+    go.sxb = (long)N * RP * D;
+    go.syb = (long)N * RP * ldphi;
+    ProfScope ps(lap ? PROF_GEMM_ORB_LAP : -1, s);
+    if (launch_gemm(go, prec, false, s)) return jq_fail(JQ_EINVAL, "orbital GEMM launch rejected");
+  }
+  const float* pis[2] = {pi[0], pi[1]};
+  const float* sgs[2] = {sigma[0], sigma[1]};
+  launch_envelope(fd, elec, atoms, pis, sgs, d.envelope, env, s);
+  ProfScope ps(lap ? PROF_DET_LAP : -1, s);
+  if (launch_det(fd, Phi, ldphi, env, det, s)) return jq_fail(JQ_EINVAL, "determinant launch rejected");
+  return 0;
 }
