@@ -60,8 +60,8 @@ def run_path(spec, system, B, width, params, x0, precision, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     _lib.profile_enable(False)
-    prof = {k: _lib.profile_read(k) for k in ("gemm_layer_lap", "det_lap", "gemm_orb_lap", "pair_lap", "forward",
-                                               "local_energy")}
+    prof = {k: _lib.profile_read(k) for k in ("gemm_layer_lap", "det_lap", "gemm_orb_lap", "pair_lap", "attn_lap",
+                                               "forward", "local_energy")}
     wall = t1 - t0
     mv = mv.cpu().numpy()
     res = dict(wall=wall, ms_per_step=1e3 * wall / steps, value=B * steps / wall, prof=prof,
@@ -162,15 +162,19 @@ def main():
     log("accuracy:", json.dumps(acc_info))
 
     # --- roofline of the dominant kernel class (layer GEMMs in Laplacian mode) ---
-    fl = RF.ferminet_flops(spec, system.atoms.shape[0])
+    # achieved = algorithmic FLOPs of all layer-GEMM launches in the timed region
+    # / their summed device time (HIP events on the launching stream)
+    fl = RF.flops(spec, system.atoms.shape[0])
     mixed = res[args.mixed]
     tot_ms, n = mixed["prof"]["gemm_layer_lap"]
-    lay_per_launch = fl["gemm_layer_lap"] * B / len(spec.hidden_single)
-    achieved = lay_per_launch / (tot_ms / n * 1e-3) if n else 0.0
+    achieved = fl["gemm_layer_lap"] * B * args.steps / (tot_ms * 1e-3) if n else 0.0
     peak = RF.PEAKS["bf16"]
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None, "kernel": "gemm_kernel (FermiNet layer, Laplacian mode)",
-            "avg_launch_ms": tot_ms / max(n, 1)}
+            "frac": achieved / peak, "traffic": None,
+            "kernel": f"gemm_kernel ({spec.kind} dense layers, Laplacian mode, {args.mixed})",
+            "avg_launch_ms": tot_ms / max(n, 1), "launches": n,
+            "per_class_ms_per_step": {k: v[0] / args.steps for k, v in mixed["prof"].items()},
+            "fp32_per_class_ms_per_step": {k: v[0] / args.steps for k, v in res["fp32"]["prof"].items()}}
 
     cpu = None
     if not args.no_cpu:
@@ -180,8 +184,11 @@ def main():
         cpu = {"value": v, "unit": "walker-steps/s", "cores": cores, "kind": "port",
                "sample": f"{nc} walkers x 1 walker-step (FP32 numpy oracle, MH move + E_L), {t:.1f}s"}
 
+    arch = (f"FermiNet {len(spec.hidden_single)}x{spec.hidden_single[0]}/{spec.hidden_double[0]}"
+            if spec.kind == "ferminet" else
+            f"Psiformer L{spec.num_layers} H{spec.num_heads} D{spec.num_heads * spec.heads_dim}")
     line = {
-        "metric": "walker-steps/s incl. E_L (N2 FermiNet, BF16-mixed; FP32 beside it)",
+        "metric": f"walker-steps/s incl. E_L ({system.name} {arch}, BF16-mixed; FP32 beside it)",
         "value": mixed["value"],
         "unit": "walker-steps/s",
         "n_gpus": 1,
@@ -193,7 +200,7 @@ def main():
         "vs_baseline": None,
         "dtype": f"{args.mixed}+fp64",
         "data": "synthetic (seeded random-init params, Gaussian walkers)",
-        "config": {"workload": f"{system.name} FermiNet 4x256/32 K={spec.ndets}", "walkers": B,
+        "config": {"workload": f"{system.name} {arch} K={spec.ndets}", "walkers": B,
                    "electrons": spec.n_elec, "mh_width": width},
         "fp32_value": res["fp32"]["value"],
         "fp32_ms_per_step": res["fp32"]["ms_per_step"],

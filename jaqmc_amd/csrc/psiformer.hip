@@ -68,81 +68,88 @@ __global__ void psi_features_kernel(FermiDims d, const float* __restrict__ elec,
 }
 
 // LayerNorm forward Laplacian (flax nn.LayerNorm, epsilon 1e-5;
-// backbone/psiformer.py:74,85).  One workgroup per unit holds the unit's
-// Rp x D tangent tile in LDS.  With d = x - mean(x) (linear), v = mean(d0^2),
-// s = (v + eps)^-1/2:
+// backbone/psiformer.py:74,85).  One workgroup per unit streams the unit's
+// Rp x D tangent tile from L2: thread r owns tangent row r for the row
+// statistics (mean, <d_r, d_0>, |d_r|^2), then thread f owns feature f for
+// the output and the cross term.  With d = x - mean(x) (linear),
+// v = mean(d0^2), s = (v + eps)^-1/2:
 //   J_v = 2/D sum_f d0 J_d,     lap_v = 2/D sum_f (d0 lap_d + sum_c J_d^2)
 //   J_s = s' J_v,                lap_s = s' lap_v + s'' sum_c J_v^2
 //   y = scale (d s) + bias:  J_y = scale (J_d s + d0 J_s),
 //   lap_y = scale (lap_d s + d0 lap_s + 2 sum_c J_d J_s)
-__global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, int D, const float* __restrict__ x,
+template <int D>
+__global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const float* __restrict__ x,
                                                             float* __restrict__ y, const float* __restrict__ scale,
                                                             const float* __restrict__ bias, float eps) {
-  extern __shared__ float sm[];
-  float* T = sm;              // Rp x D
-  float* mu = T + Rp * D;     // Rp
-  float* dot0 = mu + Rp;      // Rp
-  float* sq = dot0 + Rp;      // Rp
-  float* Js = sq + Rp;        // Rp
-  float* scal = Js + Rp;      // 4
+  __shared__ float d0s[D];
+  __shared__ float mu[256], js[256], red[8];
   const long base = (long)blockIdx.x * Rp * D;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int idx = tid; idx < Rp * D; idx += 256) T[idx] = x[base + idx];
-  __syncthreads();
-  for (int r = wave; r < Rp; r += 4) {
-    float s = 0.f;
-    for (int f = lane; f < D; f += 64) s += T[r * D + f];
-    s = wave_sum(s);
-    if (lane == 0) mu[r] = s / D;
+  const int tid = threadIdx.x;
+  const float* xu = x + base;
+  // row means (thread per row); row 0 first so every thread can centre it
+  float m = 0.f;
+  if (tid < Rp) {
+    const float4* row = (const float4*)(xu + (long)tid * D);
+#pragma unroll 4
+    for (int q = 0; q < D / 4; ++q) {
+      const float4 v = row[q];
+      m += (v.x + v.y) + (v.z + v.w);
+    }
+    m *= 1.f / D;
+    mu[tid] = m;
   }
   __syncthreads();
-  for (int idx = tid; idx < Rp * D; idx += 256) T[idx] -= mu[idx / D];
+  for (int f = tid; f < D; f += 256) d0s[f] = xu[f] - mu[0];
   __syncthreads();
-  for (int r = wave; r < Rp; r += 4) {
-    float s0 = 0.f, s1 = 0.f;
-    for (int f = lane; f < D; f += 64) {
-      const float dr = T[r * D + f];
-      s0 += T[f] * dr;
-      s1 += dr * dr;
-    }
-    s0 = wave_sum(s0);
-    s1 = wave_sum(s1);
-    if (lane == 0) {
-      dot0[r] = s0;
-      sq[r] = s1;
+  // <d_r, d_0> and |d_r|^2 per row
+  float dot = 0.f, sq = 0.f;
+  if (tid < Rp) {
+    const float4* row = (const float4*)(xu + (long)tid * D);
+#pragma unroll 4
+    for (int q = 0; q < D / 4; ++q) {
+      const float4 v = row[q];
+      const float a0 = v.x - m, a1 = v.y - m, a2 = v.z - m, a3 = v.w - m;
+      dot += a0 * d0s[4 * q] + a1 * d0s[4 * q + 1] + a2 * d0s[4 * q + 2] + a3 * d0s[4 * q + 3];
+      sq += a0 * a0 + a1 * a1 + a2 * a2 + a3 * a3;
     }
   }
+  // unit scalars: v, lap_v, J_v (per row), s, s', s''
+  const bool jrow = tid >= ROW_J0 && tid < R;
+  float p_lap = jrow ? sq : 0.f;  // sum_c |J_d|^2
+  if (tid == ROW_LAP) p_lap += dot;
+  p_lap = wave_sum(p_lap);
+  if ((tid & 63) == 0) red[tid >> 6] = p_lap;
+  if (tid == 0) red[4] = dot;  // <d0, d0>
   __syncthreads();
-  if (tid == 0) {
-    const float v = dot0[0] / D;
-    float lapv = dot0[1];
-    for (int r = ROW_J0; r < R; ++r) lapv += sq[r];
-    lapv *= 2.f / D;
-    const float ve = v + eps;
-    const float s = rsqrtf(ve);
-    const float s1 = -0.5f * s / ve;
-    const float s2 = 0.75f * s / (ve * ve);
-    float jv2 = 0.f;
-    for (int r = ROW_J0; r < R; ++r) {
-      const float jv = 2.f * dot0[r] / D;
-      Js[r] = s1 * jv;
-      jv2 += jv * jv;
-    }
-    scal[0] = s;
-    scal[1] = s1 * lapv + s2 * jv2;  // lap_s
-  }
+  const float v = red[4] / D;
+  const float lapv = 2.f / D * (red[0] + red[1] + red[2] + red[3]);
+  const float ve = v + eps;
+  const float sv = rsqrtf(ve);
+  const float s1 = -0.5f * sv / ve;
+  const float s2 = 0.75f * sv / (ve * ve);
+  const float jv = jrow ? 2.f / D * dot : 0.f;
+  float p_jv2 = wave_sum(jv * jv);
   __syncthreads();
-  for (int r = tid; r < Rp; r += 256)
-    if (r < ROW_J0 || r >= R) Js[r] = 0.f;
+  if ((tid & 63) == 0) red[tid >> 6] = p_jv2;
+  js[tid] = s1 * jv;
   __syncthreads();
-  const float s = scal[0], laps = scal[1];
+  const float laps = s1 * lapv + s2 * (red[0] + red[1] + red[2] + red[3]);
+  // output: thread per feature, rows streamed
+  float* yu = y + base;
   for (int f = tid; f < D; f += 256) {
-    const float g = scale[f], d0 = T[f];
+    const float g = scale[f], d0 = d0s[f];
     float cross = 0.f;
-    for (int r = ROW_J0; r < R; ++r) cross += T[r * D + f] * Js[r];
-    y[base + f] = g * d0 * s + bias[f];
-    if (Rp > 1) y[base + D + f] = g * (T[D + f] * s + d0 * laps + 2.f * cross);
-    for (int r = ROW_J0; r < Rp; ++r) y[base + (long)r * D + f] = r < R ? g * (T[r * D + f] * s + d0 * Js[r]) : 0.f;
+    for (int r = ROW_J0; r < R; ++r) {
+      const float dr = xu[(long)r * D + f] - mu[r];
+      cross += dr * js[r];
+      yu[(long)r * D + f] = g * (dr * sv + d0 * js[r]);
+    }
+    for (int r = R > ROW_J0 ? R : ROW_J0; r < Rp; ++r) yu[(long)r * D + f] = 0.f;
+    yu[f] = g * d0 * sv + bias[f];
+    if (Rp > 1) {
+      const float dl = xu[D + f] - mu[1];
+      yu[D + f] = g * (dl * sv + d0 * laps + 2.f * cross);
+    }
   }
 }
 
@@ -253,16 +260,14 @@ int launch_psi_features(const FermiDims& d, const float* elec, const float* atom
 
 int launch_layernorm_lap(int units, int Rp, int R, int D, const float* x, float* y, const float* scale,
                          const float* bias, float eps, hipStream_t s) {
-  const size_t smem = ((size_t)Rp * D + 4 * Rp + 4) * 4;
-  if (smem > 160 * 1024) return -1;
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)layernorm_lap_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    attr = true;
+  if (Rp > 256) return -1;
+  switch (D) {
+    case 64: hipLaunchKernelGGL(layernorm_lap_kernel<64>, dim3(units), dim3(256), 0, s, Rp, R, x, y, scale, bias, eps); return 0;
+    case 128: hipLaunchKernelGGL(layernorm_lap_kernel<128>, dim3(units), dim3(256), 0, s, Rp, R, x, y, scale, bias, eps); return 0;
+    case 256: hipLaunchKernelGGL(layernorm_lap_kernel<256>, dim3(units), dim3(256), 0, s, Rp, R, x, y, scale, bias, eps); return 0;
+    case 512: hipLaunchKernelGGL(layernorm_lap_kernel<512>, dim3(units), dim3(256), 0, s, Rp, R, x, y, scale, bias, eps); return 0;
   }
-  hipLaunchKernelGGL(layernorm_lap_kernel, dim3(units), dim3(256), smem, s, Rp, R, D, x, y, scale, bias, eps);
-  return 0;
+  return -1;
 }
 
 int launch_softmax_lap(const SoftmaxArgs& a, hipStream_t s) {

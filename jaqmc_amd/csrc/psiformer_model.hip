@@ -55,8 +55,8 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
   H = d.num_heads;
   dh = d.heads_dim;
   D = H * dh;
-  if (L < 1 || H < 1 || dh < 4 || dh % 32 != 0 || d.n_mlp < 0 || d.n_mlp > 3)
-    return jq_fail(JQ_EINVAL, "unsupported Psiformer shape (heads_dim must be a multiple of 32)");
+  if (L < 1 || H < 1 || dh < 4 || dh % 32 != 0 || d.n_mlp < 0 || d.n_mlp > 3 || (D != 64 && D != 128 && D != 256 && D != 512))
+    return jq_fail(JQ_EINVAL, "unsupported Psiformer shape (heads_dim a multiple of 32, model dim 64/128/256/512)");
   if (Rp > 128) return jq_fail(JQ_EINVAL, "Psiformer supports N <= 42 electrons (tangent tile <= 128 rows)");
   Reader rd{packed, n_floats};
   const int fin = 4 * M + 1;
@@ -238,6 +238,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     g.n_up = n_up;
     g.n_dn = n_dn;
     g.wmaj = 1;
+    ProfScope ps(lap ? PROF_GEMM_LAYER_LAP : -1, s);
     return launch_gemm(g, prec, false, s);
   };
 
@@ -268,6 +269,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       g.hs_dh = dh;
       g.hs_D = D;
       g.hs_N = N;
+      ProfScope ps(lap ? PROF_GEMM_LAYER_LAP : -1, s);
       rc |= launch_gemm(g, prec, false, s);
     }
     // ---- attention ----

@@ -60,3 +60,30 @@ def ferminet_forward_flops(spec: ModelSpec, n_atoms: int):
         f1 = d1
     tot += 2.0 * N * f1 * K * N
     return {"mfma": tot, "fp64": K * (2.0 / 3.0) * N ** 3}
+
+
+def psiformer_flops(spec: ModelSpec, n_atoms: int):
+    """Per-walker FLOPs of the Psiformer Laplacian pass, by launch class.
+
+    Dense layers on C tangent rows (input layer Local1: 5 rows); attention
+    contractions 12 N^2 C D per layer (scores, their Jacobian/Laplacian cross
+    terms, and a.v with its product-rule terms)."""
+    N = spec.n_elec
+    C = 3 * N + 2
+    D = spec.num_heads * spec.heads_dim
+    K = spec.ndets
+    layer = 2.0 * N * (4 * n_atoms + 1) * D * 5
+    attn = 0.0
+    for _ in range(spec.num_layers):
+        layer += 2.0 * N * C * D * 3 * D + 2.0 * N * C * D * D
+        dims = [D] + list(spec.mlp_hidden) + [D]
+        for a, b in zip(dims[:-1], dims[1:]):
+            layer += 2.0 * N * C * a * b
+        attn += 12.0 * N * N * C * D
+    orb = 2.0 * N * D * K * N * C
+    det = K * (6.0 * N ** 4 + (2.0 / 3.0) * N ** 3 + 2.0 * N ** 3)
+    return {"gemm_layer_lap": layer, "attn_lap": attn, "gemm_orb_lap": orb, "det_lap": det}
+
+
+def flops(spec: ModelSpec, n_atoms: int):
+    return ferminet_flops(spec, n_atoms) if spec.kind == "ferminet" else psiformer_flops(spec, n_atoms)
