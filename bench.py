@@ -112,6 +112,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256, help="walkers in the CPU-baseline sample")
     ap.add_argument("--err-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-jobs", type=int, default=12, help="jobs in the serving sample (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate of the serving sample")
+    ap.add_argument("--serving-steps", type=int, default=20, help="evaluation iterations per serving job")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="counter-calibration run for rocprofv3 --pmc: one 1 GiB device copy, then one local-energy "
+                         "pass of the workload in the mixed precision; prints nothing on stdout")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -129,6 +135,18 @@ def main():
     system, spec, B, S_steps, width = S.workload(args.workload)
     params = P.init_params(spec, seed=0)
     x0 = S.init_walkers(system, B, seed=0)
+    if args.traffic_probe:
+        from jaqmc_amd.device import DeviceModel
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB read + 1 GiB written
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        dm = DeviceModel(spec, system.atoms, system.charges, params, precision=args.mixed)
+        dm.local_energy(torch.from_numpy(x0).cuda())
+        torch.cuda.synchronize()
+        log("traffic probe done: copy 1 GiB, one local-energy pass")
+        return
     log(f"workload {args.workload}: N={spec.n_elec} M={system.atoms.shape[0]} K={spec.ndets} B={B} "
         f"device={torch.cuda.get_device_name(0)}")
 
@@ -187,6 +205,23 @@ def main():
     arch = (f"FermiNet {len(spec.hidden_single)}x{spec.hidden_single[0]}/{spec.hidden_double[0]}"
             if spec.kind == "ferminet" else
             f"Psiformer L{spec.num_layers} H{spec.num_heads} D{spec.num_heads * spec.heads_dim}")
+    serving_info = None
+    if args.serving_jobs > 0:
+        from jaqmc_amd import serving
+
+        jobs = serving.make_job_stream(args.serving_jobs, rate=args.serving_rate, seed=1, steps=args.serving_steps,
+                                       walkers=(256, 1024))
+        t0 = time.perf_counter()
+        for j in jobs:
+            j.arrival += t0
+        srv = serving.Server(lambda j: serving.GpuRunner(j), mem_budget=200 << 30, max_active=8)
+        res_s = srv.run(jobs)
+        serving_info = serving.latency_summary(res_s)
+        serving_info.update({"rate_jobs_per_s": args.serving_rate, "steps_per_job": args.serving_steps,
+                             "electrons": "uniform 2..42", "walkers": [256, 1024], "peak_active": srv.peak_active,
+                             "model": "FermiNet 4x256/32 K=16, fp32"})
+        log("serving:", json.dumps(serving_info))
+
     line = {
         "metric": f"walker-steps/s incl. E_L ({system.name} {arch}, BF16-mixed; FP32 beside it)",
         "value": mixed["value"],
@@ -208,6 +243,7 @@ def main():
         "accuracy": acc_info,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "serving": serving_info,
     }
     print(json.dumps(line), flush=True)
 

@@ -252,6 +252,179 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
 }
 
 // ---------------------------------------------------------------------------
+// Larger systems (16 < N <= 64): the Laplacian traces with FP64 MFMA.
+// Gauss-Jordan as in det_kernel; then for each coordinate c the workgroup
+// stages J_c (FP64) in LDS, its four waves form the (NP/16)^2 tiles of
+// B_c = A^-1 J_c with v_mfma_f64_16x16x4f64, write them to LDS, and all
+// threads accumulate tr(B_c) (gradient) and tr(B_c^2) (Laplacian term).
+// MFMA operand layout (16x16x4, f64): lane (r = lane%16, g = lane/16) supplies
+// A[r][4kk + g] and B[4kk + g][r]; result element v of the lane is row g + 4v,
+// column r (this differs from the f32 16x16x4 result, rows 4g + v; checked on
+// the device element by element).
+// ---------------------------------------------------------------------------
+template <int NP>
+__global__ __launch_bounds__(256) void det_mfma_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+                                                       const float* __restrict__ env, double* __restrict__ out) {
+  extern __shared__ double sm[];
+  double* A = sm;              // NP*NP  (A, later J_c)
+  double* Inv = A + NP * NP;   // NP*NP
+  double* Bb = Inv + NP * NP;  // NP*NP  (B_c tiles)
+  double* red = Bb + NP * NP;  // 64 + T
+  int* ipiv = (int*)(red + 64 + 3 * 64);
+  const int N = d.N, T = 3 * N;
+  const int w = blockIdx.x / d.K, k = blockIdx.x % d.K;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int Rp = d.Rp;
+
+  for (int idx = tid; idx < NP * NP; idx += 256) {
+    const int i = idx / NP, o = idx % NP;
+    double a = (i == o) ? 1.0 : 0.0;
+    if (i < N && o < N) {
+      const int u = unit_idx(d, w, i);
+      a = (double)Phi[(long)u * Rp * ldphi + k * N + o] * (double)env_ptr(d, env, w, i, k, o)[0];
+    }
+    A[idx] = a;
+    Inv[idx] = (i == o) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  double logdet = 0.0;
+  int sgn = 1;
+  for (int p = 0; p < NP; ++p) {
+    if (tid < 64) {
+      double best = -1.0;
+      int bi = p;
+      for (int i = p + tid; i < NP; i += 64) {
+        const double v = fabs(A[i * NP + p]);
+        if (v > best) {
+          best = v;
+          bi = i;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (tid == 0) ipiv[0] = bi;
+    }
+    __syncthreads();
+    const int q = ipiv[0];
+    if (q != p) {
+      for (int c = tid; c < NP; c += 256) {
+        double t = A[p * NP + c];
+        A[p * NP + c] = A[q * NP + c];
+        A[q * NP + c] = t;
+        t = Inv[p * NP + c];
+        Inv[p * NP + c] = Inv[q * NP + c];
+        Inv[q * NP + c] = t;
+      }
+      sgn = -sgn;
+    }
+    __syncthreads();
+    const double pv = A[p * NP + p];
+    logdet += log(fabs(pv));
+    if (pv < 0) sgn = -sgn;
+    const double rp = 1.0 / pv;
+    for (int i = tid; i < NP; i += 256) red[i] = (i == p) ? 0.0 : A[i * NP + p];
+    __syncthreads();
+    for (int c = tid; c < NP; c += 256) {
+      A[p * NP + c] *= rp;
+      Inv[p * NP + c] *= rp;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < NP * 2 * NP; idx += 256) {
+      const int i = idx / (2 * NP), c = idx % (2 * NP);
+      if (i == p) continue;
+      const double f = red[i];
+      if (c < NP)
+        A[i * NP + c] -= f * A[p * NP + c];
+      else
+        Inv[i * NP + (c - NP)] -= f * Inv[p * NP + (c - NP)];
+    }
+    __syncthreads();
+  }
+  double* o_ = out + (long)blockIdx.x * (3 + T);
+  for (int c = tid; c < T; c += 256) red[64 + c] = 0.0;
+
+  // tr(A^-1 L)
+  double part = 0.0;
+  for (int idx = tid; idx < N * N; idx += 256) {
+    const int i = idx / N, o = idx % N;
+    const int u = unit_idx(d, w, i);
+    const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
+    const float* e = env_ptr(d, env, w, i, k, o);
+    double L = (double)ph[ROW_LAP * ldphi] * e[0] + (double)ph[0] * e[1];
+    for (int ax = 0; ax < 3; ++ax) L += 2.0 * (double)ph[(ROW_J0 + 3 * i + ax) * ldphi] * e[2 + ax];
+    part += Inv[o * NP + i] * L;
+  }
+  part = wave_sum_d(part);
+  __syncthreads();
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  const double trAL = red[0] + red[1] + red[2] + red[3];
+
+  constexpr int NT = NP / 16;         // tiles per side
+  constexpr int TPW = (NT * NT + 3) / 4;  // tiles per wave
+  const int lr = lane & 15, lg = lane >> 4;
+  double tsum = 0.0;
+  double* J = A;
+  for (int c = 0; c < T; ++c) {
+    __syncthreads();  // previous B_c consumed, J free
+    for (int idx = tid; idx < NP * NP; idx += 256) {
+      const int i = idx / NP, o = idx % NP;
+      double j = 0.0;
+      if (i < N && o < N) {
+        const int u = unit_idx(d, w, i);
+        const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
+        const float* e = env_ptr(d, env, w, i, k, o);
+        j = (double)ph[(ROW_J0 + c) * ldphi] * e[0];
+        if (c / 3 == i) j += (double)ph[0] * e[2 + c % 3];
+      }
+      J[idx] = j;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+      const int tile = wave + 4 * t;
+      if (tile < NT * NT) {
+        const int P = tile / NT, Q = tile % NT;
+        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < NP / 4; ++kk) {
+          const double av = Inv[(16 * P + lr) * NP + 4 * kk + lg];
+          const double bv = J[(4 * kk + lg) * NP + 16 * Q + lr];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Bb[(16 * P + lg + 4 * v) * NP + 16 * Q + lr] = acc[v];
+      }
+    }
+    __syncthreads();
+    double tr = 0.0;
+    for (int idx = tid; idx < NP * NP; idx += 256) {
+      const int a = idx / NP, b = idx % NP;
+      tsum += Bb[idx] * Bb[b * NP + a];
+      if (a == b) tr += Bb[idx];
+    }
+    tr = wave_sum_d(tr);
+    if (lane == 0) atomicAdd(&red[64 + c], tr);
+  }
+  tsum = wave_sum_d(tsum);
+  __syncthreads();
+  if (lane == 0) red[wave] = tsum;
+  __syncthreads();
+  if (tid == 0) {
+    o_[0] = logdet;
+    o_[1] = (double)sgn;
+    o_[2] = trAL - (red[0] + red[1] + red[2] + red[3]);
+  }
+  for (int c = tid; c < T; c += 256) o_[3 + c] = red[64 + c];
+}
+
+// ---------------------------------------------------------------------------
 // Small systems (N <= 16): one wavefront per determinant, four per workgroup,
 // no workgroup barriers.  The augmented [A | I] (16 x 32 FP64) is reduced by
 // Gauss-Jordan with partial pivoting in the wave's LDS slice; then lane
@@ -597,6 +770,23 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
     hipLaunchKernelGGL(det_small_kernel, dim3((unsigned)((ndet + 3) / 4)), dim3(256), 0, s, d, Phi, ldphi, env, out,
                        d.Rp > 1 ? 1 : 0);
     return 0;
+  }
+  if (d.Rp > 1 && d.N <= 64) {
+    const int np = round_up(d.N, 16);
+    const size_t smem = (size_t)(3 * np * np + 64 + 3 * 64) * 8 + 16;
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)det_mfma_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (void)hipFuncSetAttribute((const void*)det_mfma_kernel<48>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (void)hipFuncSetAttribute((const void*)det_mfma_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      attr = true;
+    }
+    const dim3 grid((unsigned)(d.Bc * d.K));
+    switch (np) {
+      case 32: hipLaunchKernelGGL(det_mfma_kernel<32>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
+      case 48: hipLaunchKernelGGL(det_mfma_kernel<48>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
+      case 64: hipLaunchKernelGGL(det_mfma_kernel<64>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
+    }
   }
   int Np, cpi;
   size_t smem = det_smem_bytes(d.N, &Np, &cpi);

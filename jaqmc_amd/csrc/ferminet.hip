@@ -91,11 +91,75 @@ __global__ void ee_features_kernel(FermiDims d, int RP2, const float* __restrict
 
 // One two-electron layer: out = tanh(in W + b) (+ (in + out)/sqrt2 when
 // widths match), FermiLayers._update_double_stream (ferminet.py:57-63).
-// Thread = (pair, output feature); Local2 tanh rule over the 6 Jacobian rows.
+// Thread = one tangent row of one pair (RP2 = 8 rows: value, Laplacian,
+// d/dx_a, d/dx_b); W staged in LDS and read as broadcast float4; the Local2
+// tanh rule sums J^2 over the pair's six Jacobian rows with xor-shuffles
+// inside the 8-lane group of the pair.
+template <int RP2, int FIN, int FOUT>
+__global__ __launch_bounds__(256) void pair_layer_kernel(long npairs, const float* __restrict__ in,
+                                                         float* __restrict__ out, const float* __restrict__ W,
+                                                         const float* __restrict__ bias, int residual) {
+  __shared__ __attribute__((aligned(16))) float Ws[FIN * FOUT];
+  __shared__ float bs[FOUT];
+  for (int idx = threadIdx.x; idx < FIN * FOUT; idx += 256) Ws[idx] = W[idx];
+  for (int idx = threadIdx.x; idx < FOUT; idx += 256) bs[idx] = bias[idx];
+  __syncthreads();
+  const long rowid = (long)blockIdx.x * 256 + threadIdx.x;  // pair * RP2 + r
+  const long nrows = npairs * RP2;
+  const bool live = rowid < nrows;
+  const int r = (int)(rowid % RP2);
+  float x[FIN];
+  const float* xp = in + (live ? rowid : 0) * FIN;
+#pragma unroll
+  for (int k = 0; k < FIN; k += 4) *(float4*)&x[k] = *(const float4*)(xp + k);
+  float acc[FOUT];
+#pragma unroll
+  for (int f = 0; f < FOUT; ++f) acc[f] = (r == 0) ? bs[f] : 0.f;
+#pragma unroll 2
+  for (int k = 0; k < FIN; ++k) {
+#pragma unroll
+    for (int f = 0; f < FOUT; f += 4) {
+      const float4 w = *(const float4*)&Ws[k * FOUT + f];
+      acc[f] += x[k] * w.x;
+      acc[f + 1] += x[k] * w.y;
+      acc[f + 2] += x[k] * w.z;
+      acc[f + 3] += x[k] * w.w;
+    }
+  }
+  float* yp = out + (live ? rowid : 0) * FOUT;
+#pragma unroll
+  for (int f = 0; f < FOUT; ++f) {
+    // broadcast value and Laplacian rows of this pair; reduce J^2 over rows 2..7
+    float sq = (r >= 2) ? acc[f] * acc[f] : 0.f;
+    if (RP2 > 1) {
+      sq += __shfl_xor(sq, 1);
+      sq += __shfl_xor(sq, 2);
+      sq += __shfl_xor(sq, 4);
+    }
+    const int lane0 = (threadIdx.x & 63) & ~(RP2 - 1);
+    const float v = RP2 > 1 ? __shfl(acc[f], lane0) : acc[f];
+    const float t = tanhf(v);
+    const float d1 = 1.f - t * t;
+    float o;
+    if (r == 0)
+      o = t;
+    else if (r == 1)
+      o = d1 * acc[f] - 2.f * t * d1 * sq;
+    else
+      o = d1 * acc[f];
+    acc[f] = residual ? (o + x[f < FIN ? f : 0]) * 0.70710678118654752f : o;
+  }
+  if (live) {
+#pragma unroll
+    for (int f = 0; f < FOUT; f += 4) *(float4*)(yp + f) = *(float4*)&acc[f];
+  }
+}
+
+// Generic fallback (any widths): thread = (pair, output feature).
 template <int RP2>
-__global__ void pair_layer_kernel(long npairs, const float* __restrict__ in, int fin, float* __restrict__ out,
-                                  int fout, const float* __restrict__ W, const float* __restrict__ bias,
-                                  int residual) {
+__global__ void pair_layer_generic_kernel(long npairs, const float* __restrict__ in, int fin, float* __restrict__ out,
+                                          int fout, const float* __restrict__ W, const float* __restrict__ bias,
+                                          int residual) {
   const long total = npairs * fout;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
     const int f = idx % fout;
@@ -136,45 +200,93 @@ __global__ void pair_layer_kernel(long npairs, const float* __restrict__ in, int
 // channel s, m_s[i] = mean_{j in s} h2[j][i].  Expanded to dense rows:
 //   d/dx_i   <- mean_j (d/dx_b rows of pair (j, i))
 //   d/dx_j   <- (d/dx_a rows of pair (j, i)) / n_s,   j in s
-// Columns [col0 + nch*f2, ldx) are zero-filled (GEMM K padding).
+// One workgroup per (walker, electron) stages the column H2[w][:, i] in LDS
+// and writes the unit's whole block X[unit][0:Rp][col0:ldx) row by row;
+// columns past nch*f2 are zero-filled (GEMM K padding).
 template <typename T>
-__global__ void pair_means_kernel(FermiDims d, int RP2, const float* __restrict__ H2, int f2, T* __restrict__ X,
-                                  long ldx, int col0) {
-  const int ncol = ldx - col0;
-  const long total = (long)d.Bc * d.N * d.Rp * ncol;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    const int cc = idx % ncol;
-    const long ur = idx / ncol;
-    const int r = ur % d.Rp;
-    const int u = ur / d.Rp;
+__global__ __launch_bounds__(256) void pair_means_kernel(FermiDims d, int RP2, const float* __restrict__ H2, int f2,
+                                                         T* __restrict__ X, long ldx, int col0) {
+  extern __shared__ float col[];  // [j][RP2][f2]
+  float* part = col + d.N * RP2 * f2;  // [s][5][f2]: sums of value, lap, d/dx_i (3)
+  const int w = blockIdx.x / d.N, i = blockIdx.x % d.N;
+  const int tid = threadIdx.x;
+  const long pstride = (long)RP2 * f2;
+  const float* base = H2 + ((long)w * d.N * d.N + i) * pstride;  // pair (0, i)
+  for (int idx = tid; idx < d.N * RP2 * f2; idx += 256) {
+    const int j = idx / (RP2 * f2), q = idx % (RP2 * f2);
+    col[idx] = base[(long)j * d.N * pstride + q];
+  }
+  __syncthreads();
+  const int nrow_sum = RP2 > 1 ? 5 : 1;
+  for (int idx = tid; idx < d.nch * nrow_sum * f2; idx += 256) {
+    const int s = idx / (nrow_sum * f2), q = (idx / f2) % nrow_sum, f = idx % f2;
+    const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
+    const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
+    const int row = q < 2 ? q : 5 + (q - 2);  // value, lap, d/dx_b (rows 5..7)
+    float acc = 0.f;
+    for (int j = j0; j < j0 + ns; ++j) acc += col[(j * RP2 + row) * f2 + f];
+    part[(s * 5 + q) * f2 + f] = acc / ns;
+  }
+  __syncthreads();
+  const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
+  const int ncol = (int)(ldx - col0);
+  T* xu = X + (long)u * d.Rp * ldx + col0;
+  for (int idx = tid; idx < d.Rp * ncol; idx += 256) {
+    const int r = idx / ncol, cc = idx % ncol;
     float out = 0.f;
     if (cc < d.nch * f2) {
       const int s = cc / f2, f = cc % f2;
-      const int w = walker_of_unit(u, d.Bc, d.n_up, d.n_dn);
-      const int i = elec_of_unit(u, d.Bc, d.n_up, d.n_dn);
       const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
       const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
-      const float inv = 1.f / ns;
-      // pair (j, i): H2[w][j][i]
-      const float* base = H2 + ((long)w * d.N * d.N + i) * RP2 * f2 + f;
-      const long jstride = (long)d.N * RP2 * f2;
       if (r == ROW_VAL || r == ROW_LAP) {
-        float sacc = 0.f;
-        for (int j = j0; j < j0 + ns; ++j) sacc += base[j * jstride + r * f2];
-        out = sacc * inv;
+        out = part[(s * 5 + r) * f2 + f];
       } else if (r - ROW_J0 < 3 * d.N) {
         const int c = r - ROW_J0;
         const int e = c / 3, ax = c % 3;
-        if (e == i) {
-          float sacc = 0.f;
-          for (int j = j0; j < j0 + ns; ++j) sacc += base[j * jstride + (5 + ax) * f2];
-          out = sacc * inv;
-        } else if (e >= j0 && e < j0 + ns) {
-          out = base[e * jstride + (2 + ax) * f2] * inv;
-        }
+        if (e == i)
+          out = part[(s * 5 + 2 + ax) * f2 + f];
+        else if (e >= j0 && e < j0 + ns)
+          out = col[(e * RP2 + 2 + ax) * f2 + f] / ns;
       }
     }
-    X[(long)u * d.Rp * ldx + (long)r * ldx + col0 + cc] = from_f<T>(out);
+    xu[(long)r * ldx + cc] = from_f<T>(out);
+  }
+}
+
+// Spin-channel means of the one-electron stream, one workgroup per
+// (walker, channel): g[w][r][s*f1 + f] = mean_{i in s} X[unit(w, i)][r][f]
+// with float4 loads and stores (f1 % 4 == 0); columns >= nch*f1 of g are
+// zero-filled by the s = nch-1 workgroup.
+__global__ __launch_bounds__(256) void spin_means4_kernel(FermiDims d, const float* __restrict__ X, long ldx, int f1,
+                                                          float* __restrict__ g, long ldg) {
+  const int w = blockIdx.x / d.nch, s = blockIdx.x % d.nch;
+  const int i0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
+  const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
+  const int u0 = unit_of(w, i0, d.Bc, d.n_up, d.n_dn);  // units of (w, s) are consecutive
+  const float inv = 1.f / ns;
+  const int q4 = f1 / 4;
+  for (int idx = threadIdx.x; idx < d.Rp * q4; idx += 256) {
+    const int r = idx / q4, f = (idx % q4) * 4;
+    float4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < ns; ++k) {
+      const float4 v = *(const float4*)(X + ((long)(u0 + k) * d.Rp + r) * ldx + f);
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    }
+    acc.x *= inv;
+    acc.y *= inv;
+    acc.z *= inv;
+    acc.w *= inv;
+    *(float4*)(g + ((long)w * d.Rp + r) * ldg + s * f1 + f) = acc;
+  }
+  if (s == d.nch - 1) {
+    const int pad = (int)(ldg - d.nch * f1);
+    for (int idx = threadIdx.x; idx < d.Rp * pad; idx += 256) {
+      const int r = idx / pad, c = idx % pad;
+      g[((long)w * d.Rp + r) * ldg + d.nch * f1 + c] = 0.f;
+    }
   }
 }
 
@@ -271,32 +383,61 @@ int launch_ferminet_features(const FermiDims& d, const float* elec, const float*
   return 0;
 }
 
+template <int RP2>
+static bool pair_layer_fast(long npairs, const float* in, int fin, float* out, int fout, const float* W,
+                            const float* b, int res, hipStream_t s) {
+  const unsigned grid = (unsigned)((npairs * RP2 + 255) / 256);
+#define JQ_PL(FI, FO)                                                                                     \
+  if (fin == FI && fout == FO) {                                                                          \
+    hipLaunchKernelGGL((pair_layer_kernel<RP2, FI, FO>), dim3(grid), dim3(256), 0, s, npairs, in, out, W, b, \
+                       res);                                                                              \
+    return true;                                                                                          \
+  }
+  JQ_PL(4, 16)
+  JQ_PL(16, 16)
+  JQ_PL(4, 32)
+  JQ_PL(32, 32)
+#undef JQ_PL
+  return false;
+}
+
 int launch_pair_layer(const FermiDims& d, const float* H2in, int fin, float* H2out, int fout, const float* W,
                       const float* b, bool residual, hipStream_t s) {
   const long npairs = (long)d.Bc * d.N * d.N;
   const long total = npairs * fout;
-  if (d.Rp > 1)
-    hipLaunchKernelGGL(pair_layer_kernel<8>, grid_for(total), 256, 0, s, npairs, H2in, fin, H2out, fout, W, b,
-                       residual ? 1 : 0);
-  else
-    hipLaunchKernelGGL(pair_layer_kernel<1>, grid_for(total), 256, 0, s, npairs, H2in, fin, H2out, fout, W, b,
-                       residual ? 1 : 0);
+  const int res = residual ? 1 : 0;
+  if (d.Rp > 1) {
+    if (!pair_layer_fast<8>(npairs, H2in, fin, H2out, fout, W, b, res, s))
+      hipLaunchKernelGGL(pair_layer_generic_kernel<8>, grid_for(total), 256, 0, s, npairs, H2in, fin, H2out, fout, W,
+                         b, res);
+  } else {
+    if (!pair_layer_fast<1>(npairs, H2in, fin, H2out, fout, W, b, res, s))
+      hipLaunchKernelGGL(pair_layer_generic_kernel<1>, grid_for(total), 256, 0, s, npairs, H2in, fin, H2out, fout, W,
+                         b, res);
+  }
   return 0;
 }
 
 int launch_pair_means(const FermiDims& d, const float* H2, int f2, void* X, long ldx, int col0, bool x_bf16,
                       hipStream_t s) {
-  const long total = (long)d.Bc * d.N * d.Rp * (ldx - col0);
   const int RP2 = d.Rp > 1 ? 8 : 1;
+  const size_t smem = ((size_t)d.N * RP2 * f2 + 2 * 5 * f2) * 4;
+  if (smem > 64 * 1024) return -1;
+  const dim3 grid((unsigned)((long)d.Bc * d.N));
   if (x_bf16)
-    hipLaunchKernelGGL(pair_means_kernel<bf16>, grid_for(total), 256, 0, s, d, RP2, H2, f2, (bf16*)X, ldx, col0);
+    hipLaunchKernelGGL(pair_means_kernel<bf16>, grid, 256, smem, s, d, RP2, H2, f2, (bf16*)X, ldx, col0);
   else
-    hipLaunchKernelGGL(pair_means_kernel<float>, grid_for(total), 256, 0, s, d, RP2, H2, f2, (float*)X, ldx, col0);
+    hipLaunchKernelGGL(pair_means_kernel<float>, grid, 256, smem, s, d, RP2, H2, f2, (float*)X, ldx, col0);
   return 0;
 }
 
 int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void* g, long ldg, bool bf16io,
                       hipStream_t s) {
+  if (!bf16io && f1 % 4 == 0 && ldx % 4 == 0 && ldg % 4 == 0) {
+    hipLaunchKernelGGL(spin_means4_kernel, dim3((unsigned)((long)d.Bc * d.nch)), 256, 0, s, d, (const float*)X, ldx,
+                       f1, (float*)g, ldg);
+    return 0;
+  }
   const long total = (long)d.Bc * d.Rp * ldg;
   if (bf16io)
     hipLaunchKernelGGL(spin_means_kernel<bf16>, grid_for(total), 256, 0, s, d, (const bf16*)X, ldx, f1, (bf16*)g,

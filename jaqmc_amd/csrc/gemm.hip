@@ -462,6 +462,11 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BGemmArgs a) {
   const float* X = a.X + bw * a.sxw + bh * a.sxh;
   const float* W = a.W + bw * a.sww + bh * a.swh;
 
+  // Both tiles are staged as rows of 32 K-values: A row r, B row n (output
+  // column).  NT: a thread loads 4 consecutive k of one row (float4).  NN: a
+  // thread gathers 4 consecutive k of one column n (4 coalesced scalar loads
+  // across the wave).  Each thread then writes its 4 values with one packed
+  // store per BF16 term (or one float4).
   auto gload = [&](int k0) {
 #pragma unroll
     for (int v = 0; v < AV; ++v) {
@@ -480,11 +485,18 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BGemmArgs a) {
         pb[v] = (idx < BN * 8 && gc < a.ncols) ? *(const float4*)(W + (long)gc * a.ldw + k0 + q * 4)
                                                : float4{0.f, 0.f, 0.f, 0.f};
       } else {
-        // W is K x ncols: thread loads 4 consecutive columns of one k row
-        const int kk = idx / (BN / 4), c = (idx % (BN / 4)) * 4;
-        const int gk = k0 + kk, gc = col0 + c;
-        pb[v] = (idx < BN * 8 && gk < a.kvalid && gc < a.ncols) ? *(const float4*)(W + (long)gk * a.ldw + gc)
-                                                                  : float4{0.f, 0.f, 0.f, 0.f};
+        // idx -> (k quad q = idx / BN, column c = idx % BN): lanes walk columns
+        const int q = idx / BN, c = idx % BN;
+        const int gc = col0 + c;
+        float e[4] = {0.f, 0.f, 0.f, 0.f};
+        if (idx < BN * 8 && gc < a.ncols) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int gk = k0 + 4 * q + u;
+            if (gk < a.kvalid) e[u] = W[(long)gk * a.ldw + gc];
+          }
+        }
+        pb[v] = float4{e[0], e[1], e[2], e[3]};
       }
     }
   };
@@ -494,16 +506,18 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BGemmArgs a) {
       const int idx = tid + v * 256;
       if (idx >= BM * 8) break;
       const int r = idx >> 3, q = idx & 7;
-      const float e[4] = {pa[v].x, pa[v].y, pa[v].z, pa[v].w};
+      if constexpr (F32) {
+        *(float4*)(&As[0][r * LS + q * 4]) = pa[v];
+      } else {
+        bf16 t[4][NS];
+        split3(pa[v].x, t[0], NS);
+        split3(pa[v].y, t[1], NS);
+        split3(pa[v].z, t[2], NS);
+        split3(pa[v].w, t[3], NS);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if constexpr (F32) {
-          As[0][r * LS + q * 4 + u] = e[u];
-        } else {
-          bf16 t[NS];
-          split3(e[u], t, NS);
-#pragma unroll
-          for (int s2 = 0; s2 < NS; ++s2) As[s2][r * LS + q * 4 + u] = t[s2];
+        for (int s2 = 0; s2 < NS; ++s2) {
+          bf16 h[4] = {t[0][s2], t[1][s2], t[2][s2], t[3][s2]};
+          *(uint2*)(&As[s2][r * LS + q * 4]) = *(uint2*)h;
         }
       }
     }
@@ -511,24 +525,26 @@ __global__ __launch_bounds__(256) void bgemm_kernel(const BGemmArgs a) {
     for (int v = 0; v < BV; ++v) {
       const int idx = tid + v * 256;
       if (idx >= BN * 8) break;
-      const float e[4] = {pb[v].x, pb[v].y, pb[v].z, pb[v].w};
+      int c, q;
+      if constexpr (!NN) {
+        c = idx >> 3;
+        q = idx & 7;
+      } else {
+        q = idx / BN;
+        c = idx % BN;
+      }
+      if constexpr (F32) {
+        *(float4*)(&Bs[0][c * LS + q * 4]) = pb[v];
+      } else {
+        bf16 t[4][NS];
+        split3(pb[v].x, t[0], NS);
+        split3(pb[v].y, t[1], NS);
+        split3(pb[v].z, t[2], NS);
+        split3(pb[v].w, t[3], NS);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        int c, k;
-        if constexpr (!NN) {
-          c = idx >> 3;
-          k = (idx & 7) * 4 + u;
-        } else {
-          k = idx / (BN / 4);
-          c = (idx % (BN / 4)) * 4 + u;
-        }
-        if constexpr (F32) {
-          Bs[0][c * LS + k] = e[u];
-        } else {
-          bf16 t[NS];
-          split3(e[u], t, NS);
-#pragma unroll
-          for (int s2 = 0; s2 < NS; ++s2) Bs[s2][c * LS + k] = t[s2];
+        for (int s2 = 0; s2 < NS; ++s2) {
+          bf16 h[4] = {t[0][s2], t[1][s2], t[2][s2], t[3][s2]};
+          *(uint2*)(&Bs[s2][c * LS + q * 4]) = *(uint2*)h;
         }
       }
     }
@@ -625,12 +641,18 @@ void launch_bg(const BGemmArgs& a, hipStream_t s) {
 
 template <int PREC>
 void dispatch_bg(const BGemmArgs& a, hipStream_t s) {
-  if (a.ncols > 128)
+  if (a.rows <= 48) {
+    if (a.ncols > 128)
+      launch_bg<3, 4, PREC>(a, s);
+    else
+      launch_bg<3, 1, PREC>(a, s);
+  } else if (a.ncols > 128) {
     launch_bg<4, 4, PREC>(a, s);
-  else if (a.rows <= 64)
+  } else if (a.rows <= 64) {
     launch_bg<4, 1, PREC>(a, s);
-  else
+  } else {
     launch_bg<8, 1, PREC>(a, s);
+  }
 }
 
 int launch_bgemm(const BGemmArgs& a, int prec, hipStream_t s) {
