@@ -105,7 +105,8 @@ int jq_model_destroy(jq_model* m) {
 size_t jq_model_workspace_bytes(const jq_model* m) { return m ? m->ws_bytes : 0; }
 
 int jq_logpsi(jq_model* m, const float* elec, int B, float* logpsi, float* sign, void* stream) {
-  if (!m || !elec || !logpsi || B < 0) return jq_fail(JQ_EINVAL, "jq_logpsi: bad argument");
+  if (!m || B < 0 || (B > 0 && (!elec || !logpsi))) return jq_fail(JQ_EINVAL, "jq_logpsi: bad argument");
+  if (B == 0) return JQ_OK;
   hipStream_t s = (hipStream_t)stream;
   for (int b0 = 0; b0 < B; b0 += m->chunk_fwd) {
     const int Bc = std::min(m->chunk_fwd, B - b0);
@@ -118,7 +119,9 @@ int jq_logpsi(jq_model* m, const float* elec, int B, float* logpsi, float* sign,
 
 int jq_local_energy(jq_model* m, const float* elec, int B, double* e_kin, double* e_pot, double* e_loc,
                     float* logpsi, float* grad, void* stream) {
-  if (!m || !elec || !e_kin || !e_pot || !e_loc || B < 0) return jq_fail(JQ_EINVAL, "jq_local_energy: bad argument");
+  if (!m || B < 0 || (B > 0 && (!elec || !e_kin || !e_pot || !e_loc)))
+    return jq_fail(JQ_EINVAL, "jq_local_energy: bad argument");
+  if (B == 0) return JQ_OK;
   hipStream_t s = (hipStream_t)stream;
   float* lp = logpsi;
   if (!lp) {
@@ -137,7 +140,9 @@ int jq_local_energy(jq_model* m, const float* elec, int B, double* e_kin, double
 
 int jq_mh_steps(jq_model* m, float* elec, float* logp, int B, int n_steps, float width, const float* normals,
                 const float* uniforms, uint64_t seed, uint64_t ctr, int32_t* n_accept, void* stream) {
-  if (!m || !elec || !logp || !n_accept || B < 0 || n_steps < 0) return jq_fail(JQ_EINVAL, "jq_mh_steps: bad argument");
+  if (!m || B < 0 || n_steps < 0 || (B > 0 && (!elec || !logp || !n_accept)))
+    return jq_fail(JQ_EINVAL, "jq_mh_steps: bad argument");
+  if (B == 0 || n_steps == 0) return JQ_OK;
   hipStream_t s = (hipStream_t)stream;
   int rc = m->ensure_scratch(B);
   if (rc) return rc;
@@ -157,13 +162,13 @@ int jq_mh_steps(jq_model* m, float* elec, float* logp, int B, int n_steps, float
 }
 
 int jq_reduce_mean_var(const double* x, int B, double* out, void* stream) {
-  if (!x || !out || B < 0) return jq_fail(JQ_EINVAL, "jq_reduce_mean_var: bad argument");
+  if (!out || B < 0 || (B > 0 && !x)) return jq_fail(JQ_EINVAL, "jq_reduce_mean_var: bad argument");
   launch_reduce_mean_var(x, B, out, (hipStream_t)stream);
   return JQ_OK;
 }
 
 int jq_reduce_accepts(const int32_t* n_accept, int B, double* out, void* stream) {
-  if (!n_accept || !out || B < 0) return jq_fail(JQ_EINVAL, "jq_reduce_accepts: bad argument");
+  if (!out || B < 0 || (B > 0 && !n_accept)) return jq_fail(JQ_EINVAL, "jq_reduce_accepts: bad argument");
   launch_reduce_accepts(n_accept, B, out, (hipStream_t)stream);
   return JQ_OK;
 }

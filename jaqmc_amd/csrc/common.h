@@ -30,11 +30,13 @@ __host__ __device__ inline int unit_of(int w, int i, int Bc, int n_up, int n_dn)
 }
 __host__ __device__ inline int walker_of_unit(int u, int Bc, int n_up, int n_dn) {
   const int uu = Bc * n_up;
-  return u < uu ? u / n_up : (u - uu) / n_dn;
+  if (u < uu) return u / n_up;  // n_up > 0 whenever spin-up units exist
+  return (u - uu) / (n_dn > 0 ? n_dn : 1);
 }
 __host__ __device__ inline int elec_of_unit(int u, int Bc, int n_up, int n_dn) {
   const int uu = Bc * n_up;
-  return u < uu ? u % n_up : n_up + (u - uu) % n_dn;
+  if (u < uu) return u % n_up;
+  return n_up + (u - uu) % (n_dn > 0 ? n_dn : 1);
 }
 
 template <typename T> __device__ inline float to_f(T x);

@@ -37,6 +37,21 @@ def _ring6():
     return S.make_system("ring6", r.atoms[:6] * 0.5, [1.0] * 6, (3, 3))
 
 
+def _hydrogen():
+    return S.make_system("H", [[0.0, 0.0, 0.0]], [1.0], (1, 0))
+
+
+def _polarized_li():
+    return S.make_system("Li(3,0)", [[0.0, 0.0, 0.0]], [3.0], (3, 0))
+
+
+EDGE_CASES = {
+    # the reference's hydrogen demo shape: one electron, one spin channel
+    "h_atom": (_hydrogen, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=2)),
+    # fully polarized: a single non-empty spin channel (split_nonempty_channels)
+    "li_polarized": (_polarized_li, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=3)),
+}
+
 PSI_CASES = {
     # 6-electron piece of the ring, narrow Psiformer (heads_dim multiple of 32)
     "psi_small": (_ring6, dict(num_layers=2, num_heads=2, heads_dim=32, mlp_hidden=[32], ndets=2)),
@@ -49,7 +64,13 @@ def _setup(case, B):
     key = (case, B)
     if key in _cache:
         return _cache[key]
-    if case in PSI_CASES:
+    if case in EDGE_CASES:
+        mk, kw = EDGE_CASES[case]
+        sysm, spec, *_ = S.workload("li")
+        sysm = mk()
+        spec.nspins = sysm.nspins
+        spec.n_atoms = 1
+    elif case in PSI_CASES:
         mk, kw = PSI_CASES[case]
         sysm, spec, *_ = S.workload("ring42")
         if mk is not None:
@@ -77,7 +98,7 @@ def _model(sysm, spec, p, prec="fp32", chunk=0):
 
 
 @pytest.mark.parametrize("case,B", [("li", 16), ("n2_small", 16), ("n2", 12), ("h60", 4), ("n2_iso", 8),
-                                    ("psi_small", 8), ("ring42", 2)])
+                                    ("psi_small", 8), ("ring42", 2), ("h_atom", 16), ("li_polarized", 16)])
 def test_logpsi_and_local_energy_fp32(cuda, case, B):
     import torch
 
@@ -229,3 +250,32 @@ def test_mixed_precision_modes(cuda, prec, lp_tol, med_tol):
     assert np.abs(lp.cpu().numpy() - ref["logpsi"]).max() < lp_tol
     rel = np.abs(out["e_loc"].cpu().numpy() - ref["e_loc"]) / np.maximum(1.0, np.abs(ref["e_loc"]))
     assert np.median(rel) < med_tol
+
+
+def test_empty_batch_is_a_no_op(cuda):
+    import torch
+
+    sysm, spec, p, x, _ = _setup("li", 16)
+    dm = _model(sysm, spec, p)
+    e = torch.empty(0, spec.n_elec, 3, dtype=torch.float32, device=cuda)
+    lp, sg = dm.logpsi(e)
+    out = dm.local_energy(e)
+    acc = dm.mh_steps(e, torch.empty(0, device=cuda), 3, 0.1)
+    torch.cuda.synchronize()
+    assert lp.numel() == 0 and out["e_loc"].numel() == 0 and acc.numel() == 0
+
+
+def test_psiformer_chunking_and_bf16x6(cuda):
+    """Psiformer: chunked evaluation is bit-exact, and bf16x6 stays at FP32-level error."""
+    import torch
+
+    sysm, spec, p, x, ref = _setup("psi_small", 8)
+    xe = torch.from_numpy(x).to(cuda)
+    a = _model(sysm, spec, p, chunk=3)
+    b = _model(sysm, spec, p, chunk=8)
+    c = _model(sysm, spec, p, prec="bf16x6", chunk=8)
+    oa, ob, oc = a.local_energy(xe), b.local_energy(xe), c.local_energy(xe)
+    torch.cuda.synchronize()
+    assert torch.equal(oa["e_loc"], ob["e_loc"])
+    rel = np.abs(oc["e_loc"].cpu().numpy() - ref["e_loc"]) / np.maximum(1.0, np.abs(ref["e_loc"]))
+    assert np.median(rel) < 2e-5 and rel.max() < 2e-3
