@@ -22,6 +22,12 @@ namespace {
 
 constexpr int DET_THREADS = 256;
 
+// Orders LDS accesses between the lanes of one wavefront.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Orbital-matrix element (value, and optionally Jacobian row c or Laplacian).
 __device__ inline const float* env_ptr(const FermiDims& d, const float* env, int w, int i, int k, int o) {
   return env + ((((long)w * d.N + i) * d.K + k) * d.N + o) * 5;
@@ -253,38 +259,66 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
 
 // ---------------------------------------------------------------------------
 // Larger systems (16 < N <= 64): the Laplacian traces with FP64 MFMA.
-// Gauss-Jordan as in det_kernel; then for each coordinate c the workgroup
-// stages J_c (FP64) in LDS, its four waves form the (NP/16)^2 tiles of
-// B_c = A^-1 J_c with v_mfma_f64_16x16x4f64, write them to LDS, and all
-// threads accumulate tr(B_c) (gradient) and tr(B_c^2) (Laplacian term).
+// Gauss-Jordan as in det_kernel (all four waves, LDS).  The coordinate loop
+// then runs without workgroup barriers: wave q owns coordinates c = q, q+4, ...
+// and forms every 16x16 tile of B_c = A^-1 J_c in its accumulators with
+// v_mfma_f64_16x16x4f64.  The A operand (rows of A^-1) comes from LDS; the B
+// operand, J_c = dPhi/dx_c * env, comes straight from HBM into registers, one
+// coordinate ahead of its use, and is multiplied in FP64 by the envelope
+// values env[i][o] held in LDS.  The envelope-derivative part of J_c is nonzero
+// only in row c/3 and enters as the rank-1 update B_c += A^-1[:, c/3] u^T.
+// tr(B_c) is read from the diagonal tiles and tr(B_c^2) = sum_{PQ} <B_PQ,
+// B_QP^T> from a per-wave transpose of one tile at a time in LDS.
 // MFMA operand layout (16x16x4, f64): lane (r = lane%16, g = lane/16) supplies
 // A[r][4kk + g] and B[4kk + g][r]; result element v of the lane is row g + 4v,
 // column r (this differs from the f32 16x16x4 result, rows 4g + v; checked on
 // the device element by element).
 // ---------------------------------------------------------------------------
 template <int NP>
+constexpr int det_mfma_inv_stride() {
+  return NP + 2;  // rows 2 doubles apart mod 32: conflict-free A-operand reads
+}
+constexpr int DM_TS = 17;  // transpose tile stride (doubles)
+template <int NP>
+constexpr int det_mfma_env_stride() {
+  return NP % 64 == 0 ? NP + 16 : NP % 32 == 0 ? NP + 16 : NP;  // == 16 or 48 mod 64 floats: rows g hit disjoint banks
+}
+template <int NP>
+constexpr size_t det_mfma_smem() {
+  return (size_t)((NP * NP > 4 * 16 * DM_TS ? NP * NP : 4 * 16 * DM_TS) + NP * det_mfma_inv_stride<NP>() + 8) * 8 +
+         (size_t)NP * det_mfma_env_stride<NP>() * 4 + 16;
+}
+
+template <int NP>
 __global__ __launch_bounds__(256) void det_mfma_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
                                                        const float* __restrict__ env, double* __restrict__ out) {
+  constexpr int IS = det_mfma_inv_stride<NP>();
+  constexpr int NT = NP / 16, KK = NP / 4;
+  constexpr int AREG = NP * NP > 4 * 16 * DM_TS ? NP * NP : 4 * 16 * DM_TS;
   extern __shared__ double sm[];
-  double* A = sm;              // NP*NP  (A, later J_c)
-  double* Inv = A + NP * NP;   // NP*NP
-  double* Bb = Inv + NP * NP;  // NP*NP  (B_c tiles)
-  double* red = Bb + NP * NP;  // 64 + T
-  int* ipiv = (int*)(red + 64 + 3 * 64);
+  double* A = sm;           // NP*NP for the elimination, then the wave transpose tiles
+  double* Inv = A + AREG;   // NP*IS
+  double* red = Inv + NP * IS;
+  constexpr int ES = det_mfma_env_stride<NP>();
+  float* E0 = (float*)(red + 8);  // NP*ES envelope values env[i][o]
+  int* ipiv = (int*)(E0 + NP * ES);
   const int N = d.N, T = 3 * N;
   const int w = blockIdx.x / d.K, k = blockIdx.x % d.K;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int Rp = d.Rp;
+  const long rowstride = (long)d.Rp * ldphi;  // floats per unit
+  const long kN = (long)k * N;
 
-  for (int idx = tid; idx < NP * NP; idx += 256) {
-    const int i = idx / NP, o = idx % NP;
+  for (int e = tid; e < NP * NP; e += 256) {
+    const int i = e / NP, o = e % NP;
     double a = (i == o) ? 1.0 : 0.0;
+    float e0 = 0.f;
     if (i < N && o < N) {
-      const int u = unit_idx(d, w, i);
-      a = (double)Phi[(long)u * Rp * ldphi + k * N + o] * (double)env_ptr(d, env, w, i, k, o)[0];
+      e0 = env_ptr(d, env, w, i, k, o)[0];
+      a = (double)Phi[(long)unit_idx(d, w, i) * rowstride + kN + o] * (double)e0;
     }
-    A[idx] = a;
-    Inv[idx] = (i == o) ? 1.0 : 0.0;
+    A[e] = a;
+    E0[i * ES + o] = e0;
+    Inv[i * IS + o] = (i == o) ? 1.0 : 0.0;
   }
   __syncthreads();
   double logdet = 0.0;
@@ -317,9 +351,9 @@ __global__ __launch_bounds__(256) void det_mfma_kernel(FermiDims d, const float*
         double t = A[p * NP + c];
         A[p * NP + c] = A[q * NP + c];
         A[q * NP + c] = t;
-        t = Inv[p * NP + c];
-        Inv[p * NP + c] = Inv[q * NP + c];
-        Inv[q * NP + c] = t;
+        t = Inv[p * IS + c];
+        Inv[p * IS + c] = Inv[q * IS + c];
+        Inv[q * IS + c] = t;
       }
       sgn = -sgn;
     }
@@ -328,100 +362,132 @@ __global__ __launch_bounds__(256) void det_mfma_kernel(FermiDims d, const float*
     logdet += log(fabs(pv));
     if (pv < 0) sgn = -sgn;
     const double rp = 1.0 / pv;
-    for (int i = tid; i < NP; i += 256) red[i] = (i == p) ? 0.0 : A[i * NP + p];
+    // elimination factors (column p of A) go to Inv's padding column NP
+    for (int c = tid; c < NP; c += 256) Inv[c * IS + NP] = (c == p) ? 0.0 : A[c * NP + p];
     __syncthreads();
     for (int c = tid; c < NP; c += 256) {
       A[p * NP + c] *= rp;
-      Inv[p * NP + c] *= rp;
+      Inv[p * IS + c] *= rp;
     }
     __syncthreads();
     for (int idx = tid; idx < NP * 2 * NP; idx += 256) {
       const int i = idx / (2 * NP), c = idx % (2 * NP);
       if (i == p) continue;
-      const double f = red[i];
+      const double fi = Inv[i * IS + NP];
       if (c < NP)
-        A[i * NP + c] -= f * A[p * NP + c];
+        A[i * NP + c] -= fi * A[p * NP + c];
       else
-        Inv[i * NP + (c - NP)] -= f * Inv[p * NP + (c - NP)];
+        Inv[i * IS + (c - NP)] -= fi * Inv[p * IS + (c - NP)];
     }
     __syncthreads();
   }
   double* o_ = out + (long)blockIdx.x * (3 + T);
-  for (int c = tid; c < T; c += 256) red[64 + c] = 0.0;
 
-  // tr(A^-1 L)
+  // tr(A^-1 L), L = lap(phi env) = lap(phi) env + phi lap(env) + 2 grad(phi).grad(env)
   double part = 0.0;
-  for (int idx = tid; idx < N * N; idx += 256) {
-    const int i = idx / N, o = idx % N;
-    const int u = unit_idx(d, w, i);
-    const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
-    const float* e = env_ptr(d, env, w, i, k, o);
-    double L = (double)ph[ROW_LAP * ldphi] * e[0] + (double)ph[0] * e[1];
-    for (int ax = 0; ax < 3; ++ax) L += 2.0 * (double)ph[(ROW_J0 + 3 * i + ax) * ldphi] * e[2 + ax];
-    part += Inv[o * NP + i] * L;
+  for (int e = tid; e < N * N; e += 256) {
+    const int i = e / N, o = e % N;
+    const float* ph = Phi + (long)unit_idx(d, w, i) * rowstride + kN + o;
+    const float* ep = env_ptr(d, env, w, i, k, o);
+    double L = (double)ph[ROW_LAP * ldphi] * ep[0] + (double)ph[0] * ep[1];
+    for (int ax = 0; ax < 3; ++ax) L += 2.0 * (double)ph[(ROW_J0 + 3 * i + ax) * ldphi] * ep[2 + ax];
+    part += Inv[o * IS + i] * L;
   }
   part = wave_sum_d(part);
-  __syncthreads();
   if (lane == 0) red[wave] = part;
-  __syncthreads();
+  __syncthreads();  // also: A is free for the transpose tiles from here on
   const double trAL = red[0] + red[1] + red[2] + red[3];
+  __syncthreads();
 
-  constexpr int NT = NP / 16;         // tiles per side
-  constexpr int TPW = (NT * NT + 3) / 4;  // tiles per wave
+  // ---- coordinate loop, no workgroup barriers ----
   const int lr = lane & 15, lg = lane >> 4;
+  double* tile = A + wave * 16 * DM_TS;
+  int urow[KK];  // unit * Rp + ROW_J0 for the lane's matrix row m = 4kk + lg, or -1 past N
+  float jv[KK][NT];
+#pragma unroll
+  for (int kk = 0; kk < KK; ++kk) {
+    const int m = 4 * kk + lg;
+    urow[kk] = m < N ? unit_idx(d, w, m) * d.Rp + ROW_J0 : -1;
+  }
+  const float* Phik = Phi + kN;
+  auto load_j = [&](int kk, int c) {
+#pragma unroll
+    for (int Q = 0; Q < NT; ++Q) {
+      const int o = 16 * Q + lr;
+      jv[kk][Q] = (c < T && urow[kk] >= 0 && o < N) ? Phik[(long)(urow[kk] + c) * ldphi + o] : 0.f;
+    }
+  };
+#pragma unroll
+  for (int kk = 0; kk < KK; ++kk) load_j(kk, wave);
+
   double tsum = 0.0;
-  double* J = A;
-  for (int c = 0; c < T; ++c) {
-    __syncthreads();  // previous B_c consumed, J free
-    for (int idx = tid; idx < NP * NP; idx += 256) {
-      const int i = idx / NP, o = idx % NP;
-      double j = 0.0;
-      if (i < N && o < N) {
-        const int u = unit_idx(d, w, i);
-        const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
-        const float* e = env_ptr(d, env, w, i, k, o);
-        j = (double)ph[(ROW_J0 + c) * ldphi] * e[0];
-        if (c / 3 == i) j += (double)ph[0] * e[2 + c % 3];
-      }
-      J[idx] = j;
-    }
-    __syncthreads();
+  for (int c = wave; c < T; c += 4) {
+    const int cn = c + 4;
+    // envelope-derivative row: u[o] = phi[m*][o] d env[m*][o] / dx_c, m* = c / 3
+    const int ms = c / 3, ax = c % 3;
+    double u[NT];
+    {
+      const long ub = (long)unit_idx(d, w, ms) * rowstride + kN;
 #pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-      const int tile = wave + 4 * t;
-      if (tile < NT * NT) {
-        const int P = tile / NT, Q = tile % NT;
-        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kk = 0; kk < NP / 4; ++kk) {
-          const double av = Inv[(16 * P + lr) * NP + 4 * kk + lg];
-          const double bv = J[(4 * kk + lg) * NP + 16 * Q + lr];
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Bb[(16 * P + lg + 4 * v) * NP + 16 * Q + lr] = acc[v];
+      for (int Q = 0; Q < NT; ++Q) {
+        const int o = 16 * Q + lr;
+        u[Q] = o < N ? (double)Phi[ub + o] * (double)env_ptr(d, env, w, ms, k, o)[2 + ax] : 0.0;
       }
     }
-    __syncthreads();
+    f64x4 acc[NT][NT];
+#pragma unroll
+    for (int P = 0; P < NT; ++P)
+#pragma unroll
+      for (int Q = 0; Q < NT; ++Q) acc[P][Q] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      double av[NT], bv[NT];
+#pragma unroll
+      for (int P = 0; P < NT; ++P) av[P] = Inv[(16 * P + lr) * IS + 4 * kk + lg];
+#pragma unroll
+      for (int Q = 0; Q < NT; ++Q) bv[Q] = (double)jv[kk][Q] * (double)E0[(4 * kk + lg) * ES + 16 * Q + lr];
+      load_j(kk, cn);
+#pragma unroll
+      for (int P = 0; P < NT; ++P)
+#pragma unroll
+        for (int Q = 0; Q < NT; ++Q) acc[P][Q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[P], bv[Q], acc[P][Q], 0, 0, 0);
+    }
+    // rank-1 envelope term and tr(B_c)
     double tr = 0.0;
-    for (int idx = tid; idx < NP * NP; idx += 256) {
-      const int a = idx / NP, b = idx % NP;
-      tsum += Bb[idx] * Bb[b * NP + a];
-      if (a == b) tr += Bb[idx];
-    }
+#pragma unroll
+    for (int P = 0; P < NT; ++P)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double g = Inv[(16 * P + lg + 4 * v) * IS + ms];
+#pragma unroll
+        for (int Q = 0; Q < NT; ++Q) acc[P][Q][v] += g * u[Q];
+        if (lr == lg + 4 * v) tr += acc[P][P][v];
+      }
     tr = wave_sum_d(tr);
-    if (lane == 0) atomicAdd(&red[64 + c], tr);
+    if (lane == 0) o_[3 + c] = tr;
+    // tr(B_c^2): tile (P, Q) against the transpose of tile (Q, P)
+#pragma unroll
+    for (int P = 0; P < NT; ++P)
+#pragma unroll
+      for (int Q = P; Q < NT; ++Q) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) tile[(lg + 4 * v) * DM_TS + lr] = acc[Q][P][v];
+        wave_sync();
+        double s = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s += acc[P][Q][v] * tile[lr * DM_TS + lg + 4 * v];
+        tsum += (P == Q) ? s : 2.0 * s;
+        wave_sync();
+      }
   }
   tsum = wave_sum_d(tsum);
-  __syncthreads();
-  if (lane == 0) red[wave] = tsum;
+  if (lane == 0) red[4 + wave] = tsum;
   __syncthreads();
   if (tid == 0) {
     o_[0] = logdet;
     o_[1] = (double)sgn;
-    o_[2] = trAL - (red[0] + red[1] + red[2] + red[3]);
+    o_[2] = trAL - (red[4] + red[5] + red[6] + red[7]);
   }
-  for (int c = tid; c < T; c += 256) o_[3 + c] = red[64 + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -433,10 +499,6 @@ __global__ __launch_bounds__(256) void det_mfma_kernel(FermiDims d, const float*
 // tr(B_c) from the diagonal lanes and tr(B_c^2) from a transposed read of the
 // wave's B_c tile.  The next J_c is prefetched from HBM while B_c is formed.
 // ---------------------------------------------------------------------------
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
 
 constexpr int SM_NP = 16;
 struct SmallDetSmem {
@@ -773,7 +835,7 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
   }
   if (d.Rp > 1 && d.N <= 64) {
     const int np = round_up(d.N, 16);
-    const size_t smem = (size_t)(3 * np * np + 64 + 3 * 64) * 8 + 16;
+    const size_t smem = np == 32 ? det_mfma_smem<32>() : np == 48 ? det_mfma_smem<48>() : det_mfma_smem<64>();
     static bool attr = false;
     if (!attr) {
       (void)hipFuncSetAttribute((const void*)det_mfma_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

@@ -45,7 +45,18 @@ def _polarized_li():
     return S.make_system("Li(3,0)", [[0.0, 0.0, 0.0]], [3.0], (3, 0))
 
 
+def _h17():
+    return S.h_chain(17)
+
+
+def _h24():
+    return S.h_chain(24)
+
+
 EDGE_CASES = {
+    # 17 and 24 electrons: the FP64-MFMA determinant path at padded size 32
+    "h17": (_h17, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=2)),
+    "h24": (_h24, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=2)),
     # the reference's hydrogen demo shape: one electron, one spin channel
     "h_atom": (_hydrogen, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=2)),
     # fully polarized: a single non-empty spin channel (split_nonempty_channels)
@@ -69,7 +80,7 @@ def _setup(case, B):
         sysm, spec, *_ = S.workload("li")
         sysm = mk()
         spec.nspins = sysm.nspins
-        spec.n_atoms = 1
+        spec.n_atoms = sysm.atoms.shape[0]
     elif case in PSI_CASES:
         mk, kw = PSI_CASES[case]
         sysm, spec, *_ = S.workload("ring42")
@@ -98,7 +109,8 @@ def _model(sysm, spec, p, prec="fp32", chunk=0):
 
 
 @pytest.mark.parametrize("case,B", [("li", 16), ("n2_small", 16), ("n2", 12), ("h60", 4), ("n2_iso", 8),
-                                    ("psi_small", 8), ("ring42", 2), ("h_atom", 16), ("li_polarized", 16)])
+                                    ("psi_small", 8), ("ring42", 2), ("h_atom", 16), ("li_polarized", 16),
+                                    ("h17", 6), ("h24", 6)])
 def test_logpsi_and_local_energy_fp32(cuda, case, B):
     import torch
 
