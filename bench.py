@@ -26,6 +26,21 @@ import time
 import numpy as np
 
 
+def measured_traffic(workload: str):
+    """HBM bytes per dispatch of the mixed-precision layer GEMM of this workload,
+    from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated;
+    see jaqmc_amd/profiling.py), or None when no summary is committed."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"gemm_layer_lap_{workload}_traffic.txt")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_dispatch:"):
+                return float(line.split(":", 1)[1]), "profiles/" + os.path.basename(path)
+    return None, None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -188,7 +203,8 @@ def main():
     achieved = fl["gemm_layer_lap"] * B * args.steps / (tot_ms * 1e-3) if n else 0.0
     peak = RF.PEAKS["bf16"]
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None,
+            "frac": achieved / peak, "traffic": measured_traffic(args.workload)[0],
+            "traffic_unit": "bytes/launch", "traffic_source": measured_traffic(args.workload)[1],
             "kernel": f"gemm_kernel ({spec.kind} dense layers, Laplacian mode, {args.mixed})",
             "avg_launch_ms": tot_ms / max(n, 1), "launches": n,
             "per_class_ms_per_step": {k: v[0] / args.steps for k, v in mixed["prof"].items()},

@@ -62,5 +62,11 @@ __device__ inline double wave_max_d(double v) {
   return v;
 }
 
+// Orders LDS accesses between the lanes of one wavefront.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 static inline int ceil_div(long x, long m) { return (int)((x + m - 1) / m); }

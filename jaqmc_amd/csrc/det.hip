@@ -22,12 +22,6 @@ namespace {
 
 constexpr int DET_THREADS = 256;
 
-// Orders LDS accesses between the lanes of one wavefront.
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
 // Orbital-matrix element (value, and optionally Jacobian row c or Laplacian).
 __device__ inline const float* env_ptr(const FermiDims& d, const float* env, int w, int i, int k, int o) {
   return env + ((((long)w * d.N + i) * d.K + k) * d.N + o) * 5;
@@ -290,7 +284,7 @@ constexpr size_t det_mfma_smem() {
 }
 
 template <int NP>
-__global__ __launch_bounds__(256) void det_mfma_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+__global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
                                                        const float* __restrict__ env, double* __restrict__ out) {
   constexpr int IS = det_mfma_inv_stride<NP>();
   constexpr int NT = NP / 16, KK = NP / 4;

@@ -1,0 +1,763 @@
+// MFMA GEMM kernels with the forward-Laplacian activation epilogue (gfx950).
+// Included once per arithmetic mode (gemm_<mode>.hip instantiates it).
+//
+// One kernel family serves every dense layer of the hot path: the FermiNet
+// one-electron layers (value + Laplacian + 3N Jacobian rows per electron), the
+// spin-mean broadcast term, the orbital projection and the Psiformer
+// projections.  Both operands are K-contiguous FP32 in HBM and staged through
+// LDS in K-chunks of 32, with the next chunk prefetched into registers while
+// the current one is multiplied.  Each of the 4 waves owns NBW 16-column blocks
+// of the tile and *all* of its rows, so the cross-row reduction sum_c J_c^2 of
+// the tanh rule stays inside one wavefront (two xor-shuffles).
+//
+// Arithmetic:
+//   PREC_F32     v_mfma_f32_16x16x4_f32
+//   PREC_BF16    one BF16 term per operand, v_mfma_f32_16x16x32_bf16
+//   PREC_BF16X3  x = h + l (two BF16 terms): hh + hl + lh
+//   PREC_BF16X6  x = h + m + l (three BF16 terms): hh + hm + mh + hl + lh + mm,
+//                products down to 2^-16 relative are kept, so the result has
+//                FP32-level error (~2^-24) with FP32 accumulation.
+// Activations are split into BF16 terms when staged into LDS; weights are
+// constant and arrive pre-split as NS BF16 planes (GemmArgs::Wt_lo).
+//
+// K-to-lane mapping: within a 32-wide K chunk, lane group g = lane/16 owns
+// k = 8g .. 8g+7.  For the BF16 16x16x32 MFMA this is one 16-byte LDS read per
+// operand term; for the FP32 16x16x4 MFMA, step kk uses k = 8g + kk, so each
+// lane reads its 8 operands with two 16-byte LDS reads.  LDS row stride is 36
+// floats / 40 bf16, which keeps 16-byte reads of 8 consecutive rows on
+// distinct banks.
+
+#pragma once
+
+#include <type_traits>
+
+#include "kernels.h"
+
+// 0: default tile table; else UPT*10 + NBW (benchmark override)
+extern int g_gemm_variant;
+
+namespace {
+
+template <int PREC> struct NSplit { static constexpr int v = 1; };
+template <> struct NSplit<PREC_BF16X3> { static constexpr int v = 2; };
+template <> struct NSplit<PREC_BF16X6> { static constexpr int v = 3; };
+
+__device__ inline void split3(float x, bf16* t, int ns) {
+  float r = x;
+  for (int i = 0; i < ns; ++i) {
+    const bf16 h = (bf16)r;
+    t[i] = h;
+    r = r - (float)h;
+  }
+}
+
+// Workgroup -> (row tile, column block).  Workgroups are dispatched to the
+// eight XCDs round-robin (workgroup b runs on XCD b % 8, each XCD with its own
+// L2), so the column blocks of one row tile go to b, b + 8, b + 16, ...: they
+// run back to back on one XCD and the row tile's X panel is read from HBM once
+// instead of once per column block.  Row tiles past the end return at once.
+constexpr int kXcds = 8;
+__host__ __device__ inline void xcd_tile(int b, int ncb, int* rt, int* cb) {
+  const int x = b % kXcds, slot = b / kXcds;
+  *cb = slot % ncb;
+  *rt = (slot / ncb) * kXcds + x;
+}
+inline int xcd_grid(long nrt, int ncb) { return (int)((nrt + kXcds - 1) / kXcds) * kXcds * ncb; }
+
+// MBU: 16-row blocks per unit (Laplacian mode); UPT: units per tile; NBW:
+// 16-column blocks per wave.  BM = 16*MBU*UPT, BN = 64*NBW.
+template <int MBU, int UPT, int NBW, int PREC, bool LAP>
+__global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
+  constexpr int MB = MBU * UPT;
+  constexpr int BM = 16 * MB;
+  constexpr int BN = 64 * NBW;
+  constexpr int KC = 32;
+  constexpr bool F32 = PREC == PREC_F32;
+  constexpr int NS = NSplit<PREC>::v;
+  constexpr int LS = F32 ? 36 : 40;  // LDS row stride (elements)
+  using TL = typename std::conditional<F32, float, bf16>::type;
+  constexpr int NSL = F32 ? 1 : NS;
+  // LDS holds the operand tiles of one K chunk (the epilogue runs in registers)
+  constexpr int ARENA = NSL * (BM + BN) * LS * (int)sizeof(TL);
+  __shared__ __attribute__((aligned(16))) char arena[ARENA];
+  TL (*As)[BM * LS] = (TL (*)[BM * LS])arena;
+  TL (*Bs)[BN * LS] = (TL (*)[BN * LS])(arena + NSL * BM * LS * sizeof(TL));
+
+  // register prefetch: A and B chunks as float4 (8 per 32-wide row)
+  constexpr int AV = (BM * 8 + 255) / 256;
+  constexpr int BV = F32 ? (BN * 8 + 255) / 256 : 1;
+  constexpr int BVB = F32 ? 1 : (BN * 4 + 255) / 256;
+  float4 pa[AV], pb[BV];
+  uint4 pbb[NS][BVB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int wcol = wave * NBW * 16;
+  int rt, cb;
+  if (a.linear_tiles) {
+    const int ncb = (a.ncols + BN - 1) / BN;
+    rt = blockIdx.x / ncb;
+    cb = blockIdx.x % ncb;
+  } else {
+    xcd_tile(blockIdx.x, (a.ncols + BN - 1) / BN, &rt, &cb);
+  }
+  if ((long)rt * BM >= a.rows) return;
+  const long row0 = (long)rt * BM;
+  const int col0 = cb * BN;
+  const float* X = (const float*)a.X + (long)blockIdx.z * a.sxb;
+  const float* W = (const float*)a.Wt;
+
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int v = 0; v < AV; ++v) {
+      const int idx = tid + v * 256;
+      const int r = idx >> 3, q = idx & 7;
+      const long gr = row0 + r;
+      pa[v] = (idx < BM * 8 && gr < a.rows) ? *(const float4*)(X + gr * a.ldx + k0 + q * 4)
+                                             : float4{0.f, 0.f, 0.f, 0.f};
+    }
+    if constexpr (F32) {
+#pragma unroll
+      for (int v = 0; v < BV; ++v) {
+        const int idx = tid + v * 256;
+        const int c = idx >> 3, q = idx & 7;
+        const int gc = col0 + c;
+        pb[v] = (idx < BN * 8 && gc < a.ncols) ? *(const float4*)(W + (long)gc * a.ldw + k0 + q * 4)
+                                               : float4{0.f, 0.f, 0.f, 0.f};
+      }
+    } else {
+      const bf16* Wb = (const bf16*)a.Wt_lo;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int v = 0; v < BVB; ++v) {
+          const int idx = tid + v * 256;
+          const int c = idx >> 2, q = idx & 3;
+          const int gc = col0 + c;
+          pbb[s][v] = (idx < BN * 4 && gc < a.ncols)
+                          ? *(const uint4*)(Wb + s * a.wplane + (long)gc * a.ldw + k0 + q * 8)
+                          : uint4{0u, 0u, 0u, 0u};
+        }
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int v = 0; v < AV; ++v) {
+      const int idx = tid + v * 256;
+      if (idx >= BM * 8) break;
+      const int r = idx >> 3, q = idx & 7;
+      if constexpr (F32) {
+        *(float4*)(&As[0][r * LS + q * 4]) = pa[v];
+      } else {
+        bf16 t[4][NS];
+        split3(pa[v].x, t[0], NS);
+        split3(pa[v].y, t[1], NS);
+        split3(pa[v].z, t[2], NS);
+        split3(pa[v].w, t[3], NS);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          bf16 h[4] = {t[0][s], t[1][s], t[2][s], t[3][s]};
+          *(uint2*)(&As[s][r * LS + q * 4]) = *(uint2*)h;
+        }
+      }
+    }
+    if constexpr (F32) {
+#pragma unroll
+      for (int v = 0; v < BV; ++v) {
+        const int idx = tid + v * 256;
+        if (idx >= BN * 8) break;
+        const int c = idx >> 3, q = idx & 7;
+        *(float4*)(&Bs[0][c * LS + q * 4]) = pb[v];
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int v = 0; v < BVB; ++v) {
+          const int idx = tid + v * 256;
+          if (idx >= BN * 4) break;
+          const int c = idx >> 2, q = idx & 3;
+          *(uint4*)(&Bs[s][c * LS + q * 8]) = pbb[s][v];
+        }
+    }
+  };
+
+  f32x4 acc[MB][NBW];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NBW; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // Laplacian mode: the accumulators start from the broadcast term
+  // G[walker, row-in-unit, col], so the epilogue needs no extra loads.  acc[mb][nb][v] is tile row mb*16 + 4*lg + v,
+  // column wcol + nb*16 + lr.  Units are whole (rows % RU == 0), so validity
+  // is per unit; columns past ncols read a clamped address and add nothing.
+  // (For tiles taller than 128 rows the loads are deferred to the epilogue,
+  // where they do not add to the K loop's register peak.)
+  constexpr bool G_AT_INIT = MB <= 8;
+  if constexpr (LAP && G_AT_INIT) {
+    int ncl[NBW];
+    bool cin[NBW];
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) {
+      const int n = col0 + wcol + nb * 16 + lr;
+      cin[nb] = n < a.ncols;
+      ncl[nb] = cin[nb] ? n : a.ncols - 1;
+    }
+    if (a.G) {
+#pragma unroll
+      for (int u = 0; u < UPT; ++u) {
+        const long unit = row0 / (16 * MBU) + u;
+        if ((unit + 1) * (16 * MBU) <= a.rows) {
+          const long w = a.wmaj ? unit / (a.n_up + a.n_dn) : walker_of_unit((int)unit, a.Bc, a.n_up, a.n_dn);
+          const float* gu = a.G + w * (16 * MBU) * a.ldg;
+#pragma unroll
+          for (int m = 0; m < MBU; ++m) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const float* gr = gu + (long)(m * 16 + 4 * lg + v) * a.ldg;
+#pragma unroll
+              for (int nb = 0; nb < NBW; ++nb) {
+                const float g = gr[ncl[nb]];
+                acc[u * MBU + m][nb][v] = cin[nb] ? g : 0.f;
+              }
+            }
+            // at most four row blocks of loads in flight (register pressure)
+            if (m % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+      }
+    }
+  }
+
+  gload(0);
+  for (int k0 = 0; k0 < a.K; k0 += KC) {
+    lstore();
+    __syncthreads();
+    if (k0 + KC < a.K) gload(k0 + KC);
+
+    if constexpr (F32) {
+      float bf[NBW][8];
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) {
+        const float* p = &Bs[0][(wcol + nb * 16 + lr) * LS + lg * 8];
+        *(float4*)&bf[nb][0] = *(const float4*)p;
+        *(float4*)&bf[nb][4] = *(const float4*)(p + 4);
+      }
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        float af[8];
+        const float* p = &As[0][(mb * 16 + lr) * LS + lg * 8];
+        *(float4*)&af[0] = *(const float4*)p;
+        *(float4*)&af[4] = *(const float4*)(p + 4);
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+          for (int nb = 0; nb < NBW; ++nb)
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kk], bf[nb][kk], acc[mb][nb], 0, 0, 0);
+      }
+    } else {
+      bf16x8 bt[NS][NBW];
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) bt[s][nb] = *(const bf16x8*)&Bs[s][(wcol + nb * 16 + lr) * LS + lg * 8];
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        bf16x8 at[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) at[s] = *(const bf16x8*)&As[s][(mb * 16 + lr) * LS + lg * 8];
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) {
+          // small terms first
+          if constexpr (NS == 3) {
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[1][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[2][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[2], bt[0][nb], acc[mb][nb], 0, 0, 0);
+          }
+          if constexpr (NS >= 2) {
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[1][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[0][nb], acc[mb][nb], 0, 0, 0);
+          }
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0][nb], acc[mb][nb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue (registers) ----
+  // acc[mb][nb][v] holds tile row mb*16 + 4*lg + v, column wcol + nb*16 + lr.
+  // A wave owns every row of the tile for its columns, so in Laplacian mode
+  // the RU = 16*MBU rows of unit u are its row blocks u*MBU .. u*MBU+MBU-1 and
+  // the tanh rule's column sum over the Jacobian rows is an in-thread sum plus
+  // two shuffles across the lane groups.  Stores are 64-byte row segments.
+  constexpr int RU = LAP ? 16 * MBU : 1;
+  float* Y = (float*)a.Y + (long)blockIdx.z * a.syb;
+  const float* Res = (const float*)a.Res;
+  int ncol[NBW];
+  bool cok[NBW];
+#pragma unroll
+  for (int nb = 0; nb < NBW; ++nb) {
+    ncol[nb] = col0 + wcol + nb * 16 + lr;
+    cok[nb] = ncol[nb] < a.ncols;
+  }
+  // pass 1 (Laplacian mode): broadcast term (tall tiles) and bias on the value rows
+  if constexpr (LAP && !G_AT_INIT) {
+    if (a.G) {
+#pragma unroll
+      for (int u = 0; u < UPT; ++u) {
+        const long unit = row0 / RU + u;
+        if ((unit + 1) * RU <= a.rows) {
+          const long w = a.wmaj ? unit / (a.n_up + a.n_dn) : walker_of_unit((int)unit, a.Bc, a.n_up, a.n_dn);
+          const float* gu = a.G + w * RU * a.ldg;
+#pragma unroll
+          for (int m = 0; m < MBU; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const float* gr = gu + (long)(m * 16 + 4 * lg + v) * a.ldg;
+#pragma unroll
+              for (int nb = 0; nb < NBW; ++nb) {
+                const float g = gr[cok[nb] ? ncol[nb] : a.ncols - 1];
+                acc[u * MBU + m][nb][v] += cok[nb] ? g : 0.f;
+              }
+            }
+        }
+      }
+    }
+  }
+  if (LAP && a.bias) {
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) {
+      const float bv = (cok[nb] && lg == 0) ? a.bias[cok[nb] ? ncol[nb] : 0] : 0.f;
+#pragma unroll
+      for (int u = 0; u < UPT; ++u) acc[u * MBU][nb][ROW_VAL] += bv;
+    }
+  }
+  // pass 2 (Laplacian mode): tanh rule per (unit, column):
+  //   value <- t, Laplacian <- d1*lap - 2 t d1 sum_J J^2, J <- d1*J, d1 = 1 - t^2
+  if (LAP && a.act == ACT_TANH) {
+    {
+#pragma unroll
+      for (int u = 0; u < UPT; ++u)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) {
+          float sq = 0.f;
+#pragma unroll
+          for (int m = 0; m < MBU; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const float j = acc[u * MBU + m][nb][v];
+              const bool jrow = !(m == 0 && lg == 0 && v < ROW_J0);
+              sq += jrow ? j * j : 0.f;
+            }
+          sq += __shfl_xor(sq, 16);
+          sq += __shfl_xor(sq, 32);
+          const float y0 = __shfl(acc[u * MBU][nb][ROW_VAL], lr);
+          const float y1 = __shfl(acc[u * MBU][nb][ROW_LAP], lr);
+          const float t = tanhf(y0);
+          const float d1 = 1.f - t * t;
+          const float lap = d1 * y1 - 2.f * t * d1 * sq;
+#pragma unroll
+          for (int m = 0; m < MBU; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u * MBU + m][nb][v] *= d1;
+          if (lg == 0) {
+            acc[u * MBU][nb][ROW_VAL] = t;
+            acc[u * MBU][nb][ROW_LAP] = lap;
+          }
+        }
+    }
+  }
+  // pass 3: residual and store (row-major, or head-split for Psiformer Q/K/V).
+  // Each wave stages one 16-row block of its columns at a time in its own
+  // slice of the (now free) operand arena and writes it back with 16-byte
+  // accesses (16 rows of 64*NBW contiguous bytes).  The block loop is not
+  // unrolled, so the store code exists once; the staging write selects the
+  // block's accumulators with compile-time indices.
+  constexpr int WC = NBW * 16, SW = WC + 4, C4W = WC / 4;
+  static_assert(4 * 16 * SW * 4 <= ARENA, "staging slices must fit the arena");
+  __syncthreads();  // every wave is done with the operand tiles
+  float* stg = (float*)arena + wave * 16 * SW;
+  const bool vec_io = (a.ldy % 4 == 0) && (!Res || a.ldres % 4 == 0) && (a.hs_H == 0 || a.hs_dh % 4 == 0);
+#pragma unroll 1
+  for (int mb0 = 0; mb0 < MB; ++mb0) {
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+      if (mb == mb0) {
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) stg[(4 * lg + v) * SW + nb * 16 + lr] = acc[mb][nb][v];
+      }
+    wave_sync();
+#pragma unroll 1
+    for (int idx = lane; idx < 16 * C4W; idx += 64) {
+      const int r = idx / C4W, c = (idx % C4W) * 4;
+      const long grow = row0 + mb0 * 16 + r;
+      const int n = col0 + wcol + c;
+      if (grow >= a.rows || n >= a.ncols) continue;
+      float y[4];
+      *(float4*)y = *(const float4*)&stg[r * SW + c];
+      const bool tail = !(n + 4 <= a.ncols && vec_io);
+      if (!LAP) {
+        // plain mode: broadcast term, bias and activation
+        const float* gp = nullptr;
+        if (a.G) {
+          const long w = a.wmaj ? grow / (a.n_up + a.n_dn) : walker_of_unit((int)grow, a.Bc, a.n_up, a.n_dn);
+          gp = a.G + w * a.ldg + n;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (n + q < a.ncols) {
+            if (gp) y[q] += gp[q];
+            if (a.bias) y[q] += a.bias[n + q];
+          }
+          if (a.act == ACT_TANH) y[q] = tanhf(y[q]);
+        }
+      }
+      float* dst;
+      if (a.hs_H > 0) {
+        // destination [w][part][head][i][RU][dh] of walker-major unit w*hs_N + i
+        const int unit = (int)(LAP ? grow / RU : grow);
+        const int rr = LAP ? (int)(grow % RU) : 0;
+        const int w = unit / a.hs_N, i = unit % a.hs_N;
+        const int part = n / a.hs_D, hh = (n % a.hs_D) / a.hs_dh, dd = n % a.hs_dh;
+        dst = Y + ((((long)w * 3 + part) * a.hs_H + hh) * a.hs_N + i) * ((long)RU * a.hs_dh) + (long)rr * a.hs_dh + dd;
+      } else {
+        dst = Y + grow * a.ldy + n;
+        if (Res) {
+          const float* rp = Res + grow * a.ldres + n;
+          if (!tail) {
+            const float4 rq = *(const float4*)rp;
+            y[0] = (y[0] + rq.x) * a.res_scale;
+            y[1] = (y[1] + rq.y) * a.res_scale;
+            y[2] = (y[2] + rq.z) * a.res_scale;
+            y[3] = (y[3] + rq.w) * a.res_scale;
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (n + q < a.ncols) y[q] = (y[q] + rp[q]) * a.res_scale;
+          }
+        }
+      }
+      if (!tail) {
+        *(float4*)dst = *(float4*)y;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (n + q < a.ncols) dst[q] = y[q];
+      }
+    }
+    wave_sync();
+  }
+}
+
+template <int MBU, int UPT, int NBW, int PREC, bool LAP>
+void launch_one(const GemmArgs& a, hipStream_t s) {
+  constexpr int BM = 16 * MBU * UPT, BN = 64 * NBW;
+  const int ntiles = a.linear_tiles ? ceil_div(a.rows, BM) * ceil_div(a.ncols, BN)
+                                    : xcd_grid(ceil_div(a.rows, BM), ceil_div(a.ncols, BN));
+  dim3 grid(ntiles, 1, a.nbatch > 1 ? a.nbatch : 1);
+  hipLaunchKernelGGL((gemm_kernel<MBU, UPT, NBW, PREC, LAP>), grid, dim3(256), 0, s, a);
+}
+
+
+template <int MBU, int PREC>
+bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
+#define JQ_TRY(U, NB)                                                           \
+  if constexpr (MBU * U <= 12 && MBU * U * NB <= 36) {                          \
+    if (upt == U && nbw == NB) {                                                \
+      launch_one<MBU, U, NB, PREC, true>(a, s);                                 \
+      return true;                                                              \
+    }                                                                           \
+  }
+  JQ_TRY(1, 2)
+  JQ_TRY(1, 4)
+  JQ_TRY(2, 2)
+  JQ_TRY(2, 4)
+  JQ_TRY(4, 2)
+  JQ_TRY(4, 4)
+#undef JQ_TRY
+  return false;
+}
+
+// Default (UPT, NBW) per unit height and precision, from tile sweeps on
+// MI355X (K = 256-320, 256 output columns).
+void default_cfg(int R, int prec, bool wide, int* upt, int* nbw) {
+  (void)prec;
+  (void)wide;
+  *nbw = 2;
+  switch (R) {
+    case 16: *upt = 4; break;
+    case 32: *upt = 2; break;
+    case 48: *upt = 2; break;
+    case 64: *upt = 2; break;
+    default: *upt = 1; break;
+  }
+}
+
+template <int PREC>
+int dispatch(const GemmArgs& a, hipStream_t s) {
+  const bool wide = a.ncols > 128;
+  if (a.R <= 1) {
+    // two 128-column blocks share their X panel through the XCD's L2, which
+    // beats one 256-column block at a third of the occupancy
+    const bool wide_tile = g_gemm_variant > 0 ? (g_gemm_variant % 10) == 4 : false;
+    if (wide_tile)
+      launch_one<8, 1, 4, PREC, false>(a, s);
+    else
+      launch_one<8, 1, 2, PREC, false>(a, s);
+    return 0;
+  }
+  int upt, nbw;
+  default_cfg(a.R, PREC, wide, &upt, &nbw);
+  if (g_gemm_variant > 0) {
+    upt = g_gemm_variant / 10;
+    nbw = g_gemm_variant % 10;
+  }
+  bool ok = false;
+  switch (a.R) {
+    case 16: ok = launch_lap<1, PREC>(a, upt, nbw, s); break;
+    case 32: ok = launch_lap<2, PREC>(a, upt, nbw, s); break;
+    case 48: ok = launch_lap<3, PREC>(a, upt, nbw, s); break;
+    case 64: ok = launch_lap<4, PREC>(a, upt, nbw, s); break;
+    case 96: ok = launch_lap<6, PREC>(a, upt, nbw, s); break;
+    case 128: ok = launch_lap<8, PREC>(a, upt, nbw, s); break;
+    case 192: ok = launch_lap<12, PREC>(a, upt, nbw, s); break;
+    default: break;
+  }
+  return ok ? 0 : -1;
+}
+
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batched GEMM on activation operands (attention contractions).  Same K-chunk
+// pipeline and MFMA arithmetic as gemm_kernel; both operands are FP32 in HBM
+// and split into BF16 terms on the way into LDS for the BF16 modes.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int MB, int NBW, int PREC, bool NN>
+__global__ __launch_bounds__(256) void bgemm_kernel(const BGemmArgs a) {
+  constexpr int BM = 16 * MB;
+  constexpr int BN = 64 * NBW;
+  constexpr int KC = 32;
+  constexpr bool F32 = PREC == PREC_F32;
+  constexpr int NS = NSplit<PREC>::v;
+  constexpr int LS = F32 ? 36 : 40;
+  using TL = typename std::conditional<F32, float, bf16>::type;
+  constexpr int NSL = F32 ? 1 : NS;
+  __shared__ __attribute__((aligned(16))) TL As[NSL][BM * LS];
+  __shared__ __attribute__((aligned(16))) TL Bs[NSL][BN * LS];
+  constexpr int AV = (BM * 8 + 255) / 256;
+  constexpr int BV = (BN * 8 + 255) / 256;
+  float4 pa[AV], pb[BV];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int wcol = wave * NBW * 16;
+  const long row0 = (long)blockIdx.x * BM;
+  const int col0 = blockIdx.y * BN;
+  const int z = blockIdx.z;
+  const int bw = z / a.nh, bh = z % a.nh;
+  const float* X = a.X + bw * a.sxw + bh * a.sxh;
+  const float* W = a.W + bw * a.sww + bh * a.swh;
+
+  // Both tiles are staged as rows of 32 K-values: A row r, B row n (output
+  // column).  NT: a thread loads 4 consecutive k of one row (float4).  NN: a
+  // thread gathers 4 consecutive k of one column n (4 coalesced scalar loads
+  // across the wave).  Each thread then writes its 4 values with one packed
+  // store per BF16 term (or one float4).
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int v = 0; v < AV; ++v) {
+      const int idx = tid + v * 256;
+      const int r = idx >> 3, q = idx & 7;
+      const long gr = row0 + r;
+      pa[v] = (idx < BM * 8 && gr < a.rows) ? *(const float4*)(X + gr * a.ldx + k0 + q * 4)
+                                             : float4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int v = 0; v < BV; ++v) {
+      const int idx = tid + v * 256;
+      if constexpr (!NN) {
+        const int c = idx >> 3, q = idx & 7;
+        const int gc = col0 + c;
+        pb[v] = (idx < BN * 8 && gc < a.ncols) ? *(const float4*)(W + (long)gc * a.ldw + k0 + q * 4)
+                                               : float4{0.f, 0.f, 0.f, 0.f};
+      } else {
+        // idx -> (k quad q = idx / BN, column c = idx % BN): lanes walk columns
+        const int q = idx / BN, c = idx % BN;
+        const int gc = col0 + c;
+        float e[4] = {0.f, 0.f, 0.f, 0.f};
+        if (idx < BN * 8 && gc < a.ncols) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int gk = k0 + 4 * q + u;
+            if (gk < a.kvalid) e[u] = W[(long)gk * a.ldw + gc];
+          }
+        }
+        pb[v] = float4{e[0], e[1], e[2], e[3]};
+      }
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int v = 0; v < AV; ++v) {
+      const int idx = tid + v * 256;
+      if (idx >= BM * 8) break;
+      const int r = idx >> 3, q = idx & 7;
+      if constexpr (F32) {
+        *(float4*)(&As[0][r * LS + q * 4]) = pa[v];
+      } else {
+        bf16 t[4][NS];
+        split3(pa[v].x, t[0], NS);
+        split3(pa[v].y, t[1], NS);
+        split3(pa[v].z, t[2], NS);
+        split3(pa[v].w, t[3], NS);
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) {
+          bf16 h[4] = {t[0][s2], t[1][s2], t[2][s2], t[3][s2]};
+          *(uint2*)(&As[s2][r * LS + q * 4]) = *(uint2*)h;
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < BV; ++v) {
+      const int idx = tid + v * 256;
+      if (idx >= BN * 8) break;
+      int c, q;
+      if constexpr (!NN) {
+        c = idx >> 3;
+        q = idx & 7;
+      } else {
+        q = idx / BN;
+        c = idx % BN;
+      }
+      if constexpr (F32) {
+        *(float4*)(&Bs[0][c * LS + q * 4]) = pb[v];
+      } else {
+        bf16 t[4][NS];
+        split3(pb[v].x, t[0], NS);
+        split3(pb[v].y, t[1], NS);
+        split3(pb[v].z, t[2], NS);
+        split3(pb[v].w, t[3], NS);
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) {
+          bf16 h[4] = {t[0][s2], t[1][s2], t[2][s2], t[3][s2]};
+          *(uint2*)(&Bs[s2][c * LS + q * 4]) = *(uint2*)h;
+        }
+      }
+    }
+  };
+
+  f32x4 acc[MB][NBW];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NBW; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  gload(0);
+  for (int k0 = 0; k0 < a.K; k0 += KC) {
+    lstore();
+    __syncthreads();
+    if (k0 + KC < a.K) gload(k0 + KC);
+    if constexpr (F32) {
+      float bf[NBW][8];
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) {
+        const float* p = &Bs[0][(wcol + nb * 16 + lr) * LS + lg * 8];
+        *(float4*)&bf[nb][0] = *(const float4*)p;
+        *(float4*)&bf[nb][4] = *(const float4*)(p + 4);
+      }
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        float af[8];
+        const float* p = &As[0][(mb * 16 + lr) * LS + lg * 8];
+        *(float4*)&af[0] = *(const float4*)p;
+        *(float4*)&af[4] = *(const float4*)(p + 4);
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+          for (int nb = 0; nb < NBW; ++nb)
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kk], bf[nb][kk], acc[mb][nb], 0, 0, 0);
+      }
+    } else {
+      bf16x8 bt[NS][NBW];
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) bt[s2][nb] = *(const bf16x8*)&Bs[s2][(wcol + nb * 16 + lr) * LS + lg * 8];
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+        bf16x8 at[NS];
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) at[s2] = *(const bf16x8*)&As[s2][(mb * 16 + lr) * LS + lg * 8];
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) {
+          if constexpr (NS == 3) {
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[1][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[2][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[2], bt[0][nb], acc[mb][nb], 0, 0, 0);
+          }
+          if constexpr (NS >= 2) {
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[1][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[0][nb], acc[mb][nb], 0, 0, 0);
+          }
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0][nb], acc[mb][nb], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  float* Y = a.Y + bw * a.syw + bh * a.syh;
+  const float* Res = a.Res ? a.Res + bw * a.syw + bh * a.syh : nullptr;
+#pragma unroll
+  for (int nb = 0; nb < NBW; ++nb) {
+    const int n = col0 + wcol + nb * 16 + lr;
+    if (n >= a.ncols) continue;
+    const long cn = a.ycw > 0 ? (long)(n / a.ycw) * a.ycs + n % a.ycw : n;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long row = row0 + mb * 16 + lg * 4 + v;
+        if (row >= a.rows) continue;
+        float t = acc[mb][nb][v];
+        if (Res) t += Res[row * a.ldy + cn];
+        Y[row * a.ldy + cn] = t;
+      }
+  }
+}
+
+template <int MB, int NBW, int PREC>
+void launch_bg(const BGemmArgs& a, hipStream_t s) {
+  dim3 grid(ceil_div(a.rows, 16 * MB), ceil_div(a.ncols, 64 * NBW), a.nw * a.nh);
+  if (a.nn)
+    hipLaunchKernelGGL((bgemm_kernel<MB, NBW, PREC, true>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((bgemm_kernel<MB, NBW, PREC, false>), grid, dim3(256), 0, s, a);
+}
+
+template <int PREC>
+void dispatch_bg(const BGemmArgs& a, hipStream_t s) {
+  if (a.rows <= 48) {
+    if (a.ncols > 128)
+      launch_bg<3, 4, PREC>(a, s);
+    else
+      launch_bg<3, 1, PREC>(a, s);
+  } else if (a.ncols > 128) {
+    launch_bg<4, 4, PREC>(a, s);
+  } else if (a.rows <= 64) {
+    launch_bg<4, 1, PREC>(a, s);
+  } else {
+    launch_bg<8, 1, PREC>(a, s);
+  }
+}
+
+}  // namespace

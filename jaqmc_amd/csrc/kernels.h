@@ -37,6 +37,7 @@ struct GemmArgs {
   // head-split output (Psiformer QKV): column n = part*D + h*dh + d of unit
   // (w, i), row r goes to Y[(((w*3 + part)*H + h)*N + i)*R*dh + r*dh + d]
   int hs_H, hs_dh, hs_D, hs_N;
+  int linear_tiles;  // 1: plain row-tile-major workgroup order (no XCD grouping)
 };
 
 // Batched GEMM on activation operands (both FP32 in HBM, split on load for the
