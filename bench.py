@@ -81,8 +81,26 @@ def run_path(spec, system, B, width, params, x0, precision, steps, warmup):
     mv = mv.cpu().numpy()
     res = dict(wall=wall, ms_per_step=1e3 * wall / steps, value=B * steps / wall, prof=prof,
                mean=float(mv[0]), var=float(mv[1]), pmove=float(acc.sum().item()) / (B * (steps + warmup)))
+    res["x_final"] = x.cpu().numpy()
     dm.close()
     return res
+
+
+def paired_stats(a, b):
+    """Per-walker differences a - b (Hartree in, mHa out): the mean with its
+    standard error, a 1%-trimmed mean, and the spread of |a - b|.  E_L has heavy
+    tails (walkers near nodes or coalescences), so the mean alone is dominated by
+    a few walkers."""
+    d = 1e3 * (np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    n = d.size
+    keep = np.argsort(np.abs(d))[: max(1, int(np.floor(0.99 * n)))]
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+    return {"n": int(n), "mean_mHa": float(d.mean()),
+            "stderr_mHa": float(d.std(ddof=1) / np.sqrt(n)) if n > 1 else None,
+            "trimmed1pct_mean_mHa": float(d[keep].mean()),
+            "median_abs_mHa": float(np.median(np.abs(d))), "p99_abs_mHa": float(np.quantile(np.abs(d), 0.99)),
+            "max_abs_mHa": float(np.abs(d).max()), "median_rel": float(np.median(rel)),
+            "p99_rel": float(np.quantile(rel, 0.99))}
 
 
 def energies_on(spec, system, params, x, precision):
@@ -168,30 +186,33 @@ def main():
     res = {}
     for prec in ("fp32", args.mixed):
         res[prec] = run_path(spec, system, B, width, params, x0, prec, args.steps, args.warmup)
-        log(prec, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res[prec].items() if k != "prof"})
+        log(prec, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res[prec].items()
+                   if k not in ("prof", "x_final")})
         log("  prof (ms total, launches):", res[prec]["prof"])
 
-    # --- accuracy vs the FP64 oracle on a sample, and mixed vs fp32 on the full batch ---
+    # --- accuracy: FP32 and mixed E_L against the FP64 oracle on a sample, and
+    # mixed against FP32 on the full batch; on the initial walkers and on the
+    # walkers after the timed Metropolis steps ---
     from oracle import hamiltonian as ham
 
-    e32 = energies_on(spec, system, params, x0, "fp32")
-    emx = energies_on(spec, system, params, x0, args.mixed)
     ns = min(args.err_sample, B)
     p64 = P.cast_params(params, np.float64)
-    t0 = time.perf_counter()
-    ref = ham.local_energy(spec.as_oracle(), p64, x0[:ns].astype(np.float64), system.atoms.astype(np.float64),
-                           system.charges.astype(np.float64))["e_loc"]
-    log(f"FP64 oracle on {ns} walkers: {time.perf_counter() - t0:.1f}s")
-    rel = lambda a, b: np.abs(a - b) / np.maximum(np.abs(b), 1.0)
-    acc_info = {
-        "fp32_vs_fp64_sample": {"n": ns, "median_rel": float(np.median(rel(e32[:ns], ref))),
-                                "p99_rel": float(np.quantile(rel(e32[:ns], ref), 0.99)),
-                                "mean_err_mHa": float(1e3 * (e32[:ns].mean() - ref.mean()))},
-        "mixed_vs_fp64_sample": {"n": ns, "median_rel": float(np.median(rel(emx[:ns], ref))),
-                                 "p99_rel": float(np.quantile(rel(emx[:ns], ref), 0.99)),
-                                 "mean_err_mHa": float(1e3 * (emx[:ns].mean() - ref.mean()))},
-        "mixed_vs_fp32_full_batch_mean_mHa": float(1e3 * (np.mean(emx) - np.mean(e32))),
-    }
+    acc_info = {"note": "paired per-walker differences; E_L is heavy-tailed, see stderr and trimmed mean"}
+    for tag, xs in (("initial_walkers", x0), (f"after_{args.warmup + args.steps}_mh_steps",
+                                               res[args.mixed]["x_final"])):
+        e32 = energies_on(spec, system, params, xs, "fp32")
+        emx = energies_on(spec, system, params, xs, args.mixed)
+        t0 = time.perf_counter()
+        ref = ham.local_energy(spec.as_oracle(), p64, xs[:ns].astype(np.float64), system.atoms.astype(np.float64),
+                               system.charges.astype(np.float64))["e_loc"]
+        log(f"FP64 oracle on {ns} walkers ({tag}): {time.perf_counter() - t0:.1f}s")
+        acc_info[tag] = {
+            "fp32_vs_fp64_sample": paired_stats(e32[:ns], ref),
+            "mixed_vs_fp64_sample": paired_stats(emx[:ns], ref),
+            "mixed_vs_fp32_full_batch": paired_stats(emx, e32),
+            "full_batch_mean_Ha": {"fp32": float(e32.mean()), "mixed": float(emx.mean()),
+                                   "mc_stderr": float(e32.std(ddof=1) / np.sqrt(B))},
+        }
     log("accuracy:", json.dumps(acc_info))
 
     # --- roofline of the dominant kernel class (layer GEMMs in Laplacian mode) ---
@@ -230,12 +251,13 @@ def main():
         t0 = time.perf_counter()
         for j in jobs:
             j.arrival += t0
-        srv = serving.Server(lambda j: serving.GpuRunner(j), mem_budget=200 << 30, max_active=8)
+        srv = serving.Server(lambda j: serving.GpuRunner(j, precision=args.mixed), mem_budget=200 << 30,
+                             max_active=8)
         res_s = srv.run(jobs)
         serving_info = serving.latency_summary(res_s)
         serving_info.update({"rate_jobs_per_s": args.serving_rate, "steps_per_job": args.serving_steps,
                              "electrons": "uniform 2..42", "walkers": [256, 1024], "peak_active": srv.peak_active,
-                             "model": "FermiNet 4x256/32 K=16, fp32"})
+                             "model": f"FermiNet 4x256/32 K=16, {args.mixed}"})
         log("serving:", json.dumps(serving_info))
 
     line = {

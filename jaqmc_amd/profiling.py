@@ -26,7 +26,8 @@ def _short(name: str) -> str:
 def kernel_stats(db: str):
     con = sqlite3.connect(db)
     rows = list(con.execute("select name,total_calls,total_duration,average,percentage from top_kernels"))
-    return [(_short(n), int(c), t / 1e6, a / 1e6, p) for n, c, t, a, p in rows]
+    # the top_kernels view reports microseconds
+    return [(_short(n), int(c), t / 1e3, a / 1e3, p) for n, c, t, a, p in rows]
 
 
 def _per_kernel(db: str, counter: str):
