@@ -1,0 +1,345 @@
+// Fused forward-Laplacian multi-head attention (Laplacian mode), one
+// workgroup per (walker, head).
+//
+// The product rule of laplacian/primitives/core.py:494-500 applied to
+// s = q.k, a = softmax_j(s), o = a.v (backbone/psiformer.py:29-60; q carries
+// the 1/sqrt(dh) scale in its weights).  Q, K, V arrive head-split,
+// [w][part][h][i][Rp][dh]: row 0 value, row 1 Laplacian, rows 2 .. R-1 the
+// Jacobian over the 3N electron coordinates, zero rows up to Rp.  For query
+// i, key j and tangent row c:
+//   J_c[i][j] = Q_c[i].K_0[j] + Q_0[i].K_c[j]           (c >= 1; c = 1 is the
+//                                                         first-order part of lap s)
+//   lap s     = J_1 + 2 S3,  S3[i][j] = sum_{c>=2} Q_c[i].K_c[j]
+//   mu_c[i]   = sum_j a J_c,  delta_c = J_c - mu_c,  Ja_c = a delta_c
+//   lap a     = a (rj - nu + lap s - lam),  rj = sum_c delta_c^2,
+//               nu = sum_j a rj,  lam = sum_j a lap s
+//   O_0 = a V_0,  O_c = Ja_c V_0 + a V_c  (c >= 2),
+//   O_1 = lap_a V_0 + a V_1 + 2 sum_{c>=2} Ja_c V_c.
+// The workgroup stages the value rows Q_0, K_0, V_0 and the attention
+// weights once, then streams the tangent rows: for each c it stages
+// Q_c, K_c, V_c (the next c is prefetched into registers meanwhile), forms
+// the 16x16 tiles of J_c (and accumulates S3) with v_mfma_f32_16x16x4f32,
+// centres and weights them row by row (one wave per query row, lanes over
+// keys, shuffles for the sums), forms O_c (and accumulates sum_c Ja_c V_c)
+// and writes O_c straight to the attention output.  The score tensors never
+// leave LDS.
+//
+// Eight waves share each product's 16x16 output tiles.  MFMA 16x16x4 (f32):
+// lane (r = lane%16, g = lane/16) supplies A[r][k] and B[k][r]; result
+// element v is row 4g + v, column r.  Lane group g owns the contiguous K range
+// [g*K/4, (g+1)*K/4), so one 16-byte LDS read per operand feeds four MFMA
+// steps (V is staged transposed, [d][j], to make its K index j contiguous).
+// The tiles of a wave run as interleaved, independent MFMA chains.
+
+#include "kernels.h"
+
+namespace {
+
+constexpr int ATT_THREADS = 512;
+constexpr int ATT_WAVES = ATT_THREADS / 64;
+
+template <int NT, int DH>
+struct AttnLayout {
+  static constexpr int NP = 16 * NT;   // padded electrons
+  static constexpr int SQ = DH + 4;    // Q, K rows [i][d]
+  static constexpr int SJ = NP + 4;    // [i][j] arrays and transposed V [d][j]
+  static constexpr int FLOATS = 4 * NP * SQ + 2 * DH * SJ + 3 * NP * SJ;
+};
+
+template <int NT, int DH>
+__global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
+  using L = AttnLayout<NT, DH>;
+  constexpr int NP = L::NP, SQ = L::SQ, SJ = L::SJ;
+  constexpr int DT = DH / 16;                               // output column tiles
+  constexpr int JT = NT * NT, OT = NT * DT;                  // score / output tiles
+  constexpr int JTW = (JT + ATT_WAVES - 1) / ATT_WAVES, OTW = (OT + ATT_WAVES - 1) / ATT_WAVES;
+  constexpr int RPW = NP / ATT_WAVES > 0 ? (NP + ATT_WAVES - 1) / ATT_WAVES : 1;  // query rows per wave
+  constexpr int F4 = DH / 4;                                 // float4 per row
+  constexpr int PER = (3 * NP * F4 + ATT_THREADS - 1) / ATT_THREADS;
+  constexpr int KQ = DH / 4, KA = NP / 4;                    // K range per lane group (q.k / a.v)
+
+  extern __shared__ float sm[];
+  float* Q0 = sm;             // Q_0, then K_0 at + NP*SQ
+  float* K0 = Q0 + NP * SQ;
+  float* Qc = K0 + NP * SQ;   // Q_c, then K_c
+  float* Kc = Qc + NP * SQ;
+  float* V0 = Kc + NP * SQ;   // V_0 transposed [d][j]
+  float* Vc = V0 + DH * SJ;   // V_c transposed
+  float* As = Vc + DH * SJ;   // attention weights a[i][j]
+  float* Jt = As + NP * SJ;   // J_c, then Ja_c (then S3, then lap a)
+  float* J1 = Jt + NP * SJ;   // J_1
+
+  const int N = a.N, RP = a.Rp, R = a.R;
+  const int z = blockIdx.x, w = z / a.H, h = z % a.H;
+  const long blk = (long)N * RP * DH;
+  const float* qsrc = a.QKV + ((long)(w * 3) * a.H + h) * blk;  // K and V follow at + H*blk, + 2*H*blk
+  const long pstride = (long)a.H * blk;
+  float* O = a.O + (long)w * N * RP * a.ldo + (long)h * DH;  // row (i, c) at ((i*RP + c) * ldo)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lg = lane >> 4;
+
+  for (int e = tid; e < L::FLOATS; e += ATT_THREADS) sm[e] = 0.f;  // padding rows and columns stay zero
+
+  // row c of Q, K, V (rows i < N) through registers
+  float4 pf[PER];
+  auto fetch = [&](int c) __attribute__((always_inline)) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int it = tid + ATT_THREADS * p;
+      const int m = it / (NP * F4), rem = it % (NP * F4), i = rem / F4, q = rem % F4;
+      pf[p] = (m < 3 && i < N && c < R) ? *(const float4*)(qsrc + m * pstride + ((long)i * RP + c) * DH + 4 * q)
+                                        : float4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  // dq: a Q array, immediately followed by its K array; dv: a transposed V array
+  auto commit = [&](float* dq, float* dv) __attribute__((always_inline)) {
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int it = tid + ATT_THREADS * p;
+      const int m = it / (NP * F4), rem = it % (NP * F4), i = rem / F4, q = rem % F4;
+      if (m < 2 && i < N) {
+        *(float4*)(dq + (m * NP + i) * SQ + 4 * q) = pf[p];
+      } else if (m == 2 && i < N) {
+        dv[(4 * q + 0) * SJ + i] = pf[p].x;
+        dv[(4 * q + 1) * SJ + i] = pf[p].y;
+        dv[(4 * q + 2) * SJ + i] = pf[p].z;
+        dv[(4 * q + 3) * SJ + i] = pf[p].w;
+      }
+    }
+  };
+  // A row (16P + lr) / B row (16Q + lr) of an [row][k] array, four k from lane group lg's range
+  auto rd4 = [&](const float* M, int ld, int row, int kbase, int g) __attribute__((always_inline)) {
+    return *(const float4*)(M + row * ld + kbase + 4 * g);
+  };
+  auto mfma4 = [&](f32x4& acc, const float4& x, const float4& y) __attribute__((always_inline)) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, y.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, y.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, y.w, acc, 0, 0, 0);
+  };
+  auto put_tile = [&](float* D, const f32x4& acc, int P, int Qt) __attribute__((always_inline)) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) D[(16 * P + 4 * lg + v) * SJ + 16 * Qt + lr] = acc[v];
+  };
+  auto store_o = [&](const f32x4& acc, int P, int Dq, int c) __attribute__((always_inline)) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * P + 4 * lg + v;
+      if (i < N) O[((long)i * RP + c) * a.ldo + 16 * Dq + lr] = acc[v];
+    }
+  };
+  // output tiles of a.v-type products: acc[t] += A[i][:] . Vt[d][:] for the wave's tiles
+  auto av_tiles = [&](f32x4* acc, const float* A, const float* Vt) __attribute__((always_inline)) {
+#pragma unroll
+    for (int g = 0; g < KA / 4; ++g)
+#pragma unroll
+      for (int t = 0; t < OTW; ++t) {
+        const int tile = wave + ATT_WAVES * t;
+        if (tile < OT) {
+          const int P = tile / DT, Dq = tile % DT;
+          mfma4(acc[t], rd4(A, SJ, 16 * P + lr, lg * KA, g), rd4(Vt, SJ, 16 * Dq + lr, lg * KA, g));
+        }
+      }
+  };
+  const bool jok = lane < N;
+
+  // ---- value rows and the attention weights ----
+  fetch(0);
+  __syncthreads();  // LDS zeroed
+  commit(Q0, V0);
+  fetch(1);
+  __syncthreads();
+  {
+    f32x4 acc[JTW];
+#pragma unroll
+    for (int t = 0; t < JTW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < KQ / 4; ++g)
+#pragma unroll
+      for (int t = 0; t < JTW; ++t) {
+        const int tile = wave + ATT_WAVES * t;
+        if (tile < JT)
+          mfma4(acc[t], rd4(Q0, SQ, 16 * (tile / NT) + lr, lg * KQ, g), rd4(K0, SQ, 16 * (tile % NT) + lr, lg * KQ, g));
+      }
+#pragma unroll
+    for (int t = 0; t < JTW; ++t) {
+      const int tile = wave + ATT_WAVES * t;
+      if (tile < JT) put_tile(As, acc[t], tile / NT, tile % NT);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < RPW; ++t) {
+    const int i = wave + ATT_WAVES * t;
+    if (i < N) {
+      const float s = jok ? As[i * SJ + lane] : -INFINITY;
+      float m = s;
+      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+      const float e = jok ? expf(s - m) : 0.f;
+      const float zs = wave_sum(e);
+      if (lane < NP) As[i * SJ + lane] = e / zs;
+    }
+  }
+
+  // ---- tangent rows ----
+  f32x4 s3[JTW], l6[OTW];
+#pragma unroll
+  for (int t = 0; t < JTW; ++t) s3[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < OTW; ++t) l6[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  float rj[RPW];
+#pragma unroll
+  for (int t = 0; t < RPW; ++t) rj[t] = 0.f;
+
+  for (int c = 1; c < R; ++c) {
+    __syncthreads();  // previous row's readers of Qc, Kc, Vc and Jt are done
+    commit(Qc, Vc);
+    fetch(c + 1);
+    __syncthreads();
+    {
+      // J_c = Q_c K_0^T + Q_0 K_c^T  and  S3 += Q_c K_c^T
+      f32x4 acc[JTW];
+#pragma unroll
+      for (int t = 0; t < JTW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+      const bool cross = c >= ROW_J0;
+#pragma unroll
+      for (int g = 0; g < KQ / 4; ++g)
+#pragma unroll
+        for (int t = 0; t < JTW; ++t) {
+          const int tile = wave + ATT_WAVES * t;
+          if (tile < JT) {
+            const int P = tile / NT, Qt = tile % NT;
+            const float4 qc = rd4(Qc, SQ, 16 * P + lr, lg * KQ, g), q0 = rd4(Q0, SQ, 16 * P + lr, lg * KQ, g);
+            const float4 k0 = rd4(K0, SQ, 16 * Qt + lr, lg * KQ, g), kc = rd4(Kc, SQ, 16 * Qt + lr, lg * KQ, g);
+            mfma4(acc[t], qc, k0);
+            mfma4(acc[t], q0, kc);
+            if (cross) mfma4(s3[t], qc, kc);
+          }
+        }
+      float* Jdst = c == 1 ? J1 : Jt;
+#pragma unroll
+      for (int t = 0; t < JTW; ++t) {
+        const int tile = wave + ATT_WAVES * t;
+        if (tile < JT) put_tile(Jdst, acc[t], tile / NT, tile % NT);
+      }
+    }
+    if (c == 1) continue;  // V_1 enters the Laplacian row after the loop
+    __syncthreads();
+    // centre and weight: Jt <- a (J_c - mu_c), rj += (J_c - mu_c)^2
+#pragma unroll
+    for (int t = 0; t < RPW; ++t) {
+      const int i = wave + ATT_WAVES * t;
+      if (i < N) {
+        const float jv = jok ? Jt[i * SJ + lane] : 0.f;
+        const float av = jok ? As[i * SJ + lane] : 0.f;
+        const float mu = wave_sum(av * jv);
+        const float d = jok ? jv - mu : 0.f;
+        if (jok) Jt[i * SJ + lane] = av * d;
+        rj[t] += d * d;
+      }
+    }
+    __syncthreads();
+    {
+      // O_c = Ja_c V_0 + a V_c,  L6 += Ja_c V_c
+      f32x4 acc[OTW];
+#pragma unroll
+      for (int t = 0; t < OTW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int g = 0; g < KA / 4; ++g)
+#pragma unroll
+        for (int t = 0; t < OTW; ++t) {
+          const int tile = wave + ATT_WAVES * t;
+          if (tile < OT) {
+            const int P = tile / DT, Dq = tile % DT;
+            const float4 ja = rd4(Jt, SJ, 16 * P + lr, lg * KA, g), aa = rd4(As, SJ, 16 * P + lr, lg * KA, g);
+            const float4 v0 = rd4(V0, SJ, 16 * Dq + lr, lg * KA, g), vc = rd4(Vc, SJ, 16 * Dq + lr, lg * KA, g);
+            mfma4(acc[t], ja, v0);
+            mfma4(acc[t], aa, vc);
+            mfma4(l6[t], ja, vc);
+          }
+        }
+#pragma unroll
+      for (int t = 0; t < OTW; ++t) {
+        const int tile = wave + ATT_WAVES * t;
+        if (tile < OT) store_o(acc[t], tile / DT, tile % DT, c);
+      }
+    }
+  }
+
+  // ---- Laplacian row and value row of the output ----
+  __syncthreads();
+  fetch(1);  // V_1 (and Q_1, K_1, unused)
+#pragma unroll
+  for (int t = 0; t < JTW; ++t) {
+    const int tile = wave + ATT_WAVES * t;
+    if (tile < JT) put_tile(Jt, s3[t], tile / NT, tile % NT);
+  }
+  commit(Qc, Vc);
+  __syncthreads();
+  // lap s = J_1 + 2 S3; lap a = a (rj - nu + lap s - lam)
+#pragma unroll
+  for (int t = 0; t < RPW; ++t) {
+    const int i = wave + ATT_WAVES * t;
+    if (i < N) {
+      const float av = jok ? As[i * SJ + lane] : 0.f;
+      const float ls = jok ? J1[i * SJ + lane] + 2.f * Jt[i * SJ + lane] : 0.f;
+      const float r = jok ? rj[t] : 0.f;
+      const float nu = wave_sum(av * r), lam = wave_sum(av * ls);
+      if (jok) Jt[i * SJ + lane] = av * (r - nu + ls - lam);
+    }
+  }
+  __syncthreads();
+  {
+    f32x4 o0[OTW], o1[OTW];
+#pragma unroll
+    for (int t = 0; t < OTW; ++t) {
+      o0[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+      o1[t] = f32x4{2.f * l6[t][0], 2.f * l6[t][1], 2.f * l6[t][2], 2.f * l6[t][3]};
+    }
+    av_tiles(o0, As, V0);
+    av_tiles(o1, Jt, V0);
+    av_tiles(o1, As, Vc);
+#pragma unroll
+    for (int t = 0; t < OTW; ++t) {
+      const int tile = wave + ATT_WAVES * t;
+      if (tile < OT) {
+        store_o(o0[t], tile / DT, tile % DT, ROW_VAL);
+        store_o(o1[t], tile / DT, tile % DT, ROW_LAP);
+      }
+    }
+  }
+  // padding rows R .. Rp-1 of the output are zero
+  for (int e = tid; e < N * (RP - R) * DH; e += ATT_THREADS) {
+    const int d = e % DH, rc = e / DH, i = rc / (RP - R), c = R + rc % (RP - R);
+    O[((long)i * RP + c) * a.ldo + d] = 0.f;
+  }
+}
+
+template <int NT, int DH>
+int launch_one(const AttnArgs& a, hipStream_t s) {
+  const size_t smem = (size_t)AttnLayout<NT, DH>::FLOATS * 4;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)attn_lap_kernel<NT, DH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)smem);
+    attr = true;
+  }
+  hipLaunchKernelGGL((attn_lap_kernel<NT, DH>), dim3(a.nz), dim3(ATT_THREADS), smem, s, a);
+  return 0;
+}
+
+}  // namespace
+
+bool attn_lap_supported(int N, int dh) { return N >= 1 && N <= 48 && (dh == 32 || dh == 64); }
+
+int launch_attn_lap(const AttnArgs& a, hipStream_t s) {
+  if (!attn_lap_supported(a.N, a.dh) || a.nz <= 0 || a.R > a.Rp || a.R < ROW_J0 || a.ldo % 4) return -1;
+  const int nt = (a.N + 15) / 16;
+  if (a.dh == 64) {
+    if (nt == 1) return launch_one<1, 64>(a, s);
+    if (nt == 2) return launch_one<2, 64>(a, s);
+    return launch_one<3, 64>(a, s);
+  }
+  if (nt == 1) return launch_one<1, 32>(a, s);
+  if (nt == 2) return launch_one<2, 32>(a, s);
+  return launch_one<3, 32>(a, s);
+}

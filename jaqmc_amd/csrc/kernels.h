@@ -130,3 +130,14 @@ struct SoftmaxArgs {
   int N, Rp, R, nz;
 };
 int launch_softmax_lap(const SoftmaxArgs& a, hipStream_t s);
+
+// Fused forward-Laplacian attention (attention.hip): QKV head-split
+// [w][part][h][i][Rp][dh]; output row (i, c) of walker w, head h at
+// O + ((w*N + i)*Rp + c)*ldo + h*dh.  nz = walkers * heads.
+struct AttnArgs {
+  const float* QKV;
+  float* O; int ldo;
+  int N, Rp, R, H, dh, nz;
+};
+bool attn_lap_supported(int N, int dh);
+int launch_attn_lap(const AttnArgs& a, hipStream_t s);

@@ -159,28 +159,33 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
 //   a = softmax(s);  mu_c = sum_j a_j J_jc;  delta_jc = J_jc - mu_c
 //   J_a_jc = a_j delta_jc
 //   lap_a_j = a_j (sum_c delta_jc^2 - sum_k a_k sum_c delta_kc^2 + lap_j - sum_k a_k lap_k)
+//
+// One workgroup per (z, i); N <= 64 keys, one per lane.  Wave w takes the
+// Jacobian rows c = 2 + w, 6 + w, ...: lane j gathers J_jc, mu_c is a wave
+// sum, the centred delta_jc goes to LDS for the output pass and its square
+// is accumulated per lane; the four waves' partial sums give sum_c delta_jc^2.
 __global__ __launch_bounds__(256) void softmax_lap_kernel(SoftmaxArgs a) {
   extern __shared__ float sm[];
   const int N = a.N, Rp = a.Rp, R = a.R;
-  float* J = sm;              // Rp x N (rows c >= 2 used)
+  float* J = sm;              // Rp x N: delta_jc (rows c >= 2 used)
   float* av = J + Rp * N;     // N: a_j
   float* lp = av + N;         // N: lap_j
-  float* rj = lp + N;         // N
-  float* red = rj + N;        // 8
-  const int z = blockIdx.x / N, i = blockIdx.x % N;
-  const int tid = threadIdx.x;
+  float* rq = lp + N;         // 4 x 64: per-wave partial sum_c delta_jc^2, then rq[j] = total
+  float* red = rq + 256;      // 8
+  // Workgroups go to the eight XCDs round-robin; the N query rows of one
+  // (walker, head) are given to workgroups 8 apart so that they share one
+  // XCD's L2 for their strided reads of S2[z].
+  const int b = blockIdx.x, x8 = b & 7, slot = b >> 3;
+  const int z = (slot / N) * 8 + x8, i = slot % N;
+  if (z >= a.nz) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long zS = (long)z * N * Rp * a.lds;
   const float* S1 = a.S1 + zS + (long)i * Rp * a.lds;
+  const float* S2 = a.S2 + zS + i;
   for (int j = tid; j < N; j += 256) {
     av[j] = S1[j];
-    if (Rp > 1) lp[j] = S1[a.lds + j] + a.S2[zS + ((long)j * Rp + 1) * a.lds + i] +
-                        2.f * a.S3[(long)z * N * a.lds + (long)i * a.lds + j];
+    if (Rp > 1) lp[j] = S1[a.lds + j] + S2[((long)j * Rp + 1) * a.lds] + 2.f * a.S3[(long)z * N * a.lds + (long)i * a.lds + j];
   }
-  if (Rp > 1)
-    for (int idx = tid; idx < (R - ROW_J0) * N; idx += 256) {
-      const int c = ROW_J0 + idx / N, j = idx % N;
-      J[c * N + j] = S1[(long)c * a.lds + j] + a.S2[zS + ((long)j * Rp + c) * a.lds + i];
-    }
   __syncthreads();
   if (tid < 64) {
     float m = -INFINITY;
@@ -197,27 +202,27 @@ __global__ __launch_bounds__(256) void softmax_lap_kernel(SoftmaxArgs a) {
     for (int j = tid; j < a.lda1; j += 256) A1[j] = j < N ? av[j] : 0.f;
     return;
   }
-  // mu_c and delta
-  for (int c = ROW_J0 + tid; c < R; c += 256) {
-    float mu = 0.f;
-    for (int j = 0; j < N; ++j) mu += av[j] * J[c * N + j];
-    for (int j = 0; j < N; ++j) J[c * N + j] -= mu;
+  // Jacobian rows: centring by mu_c and the per-key sum of squares
+  const bool jok = lane < N;
+  const float aj = jok ? av[lane] : 0.f;
+  const int jl = jok ? lane : 0;  // clamped key index: every address stays in the buffers
+  const float* S2j = S2 + (long)jl * Rp * a.lds;
+  float rpart = 0.f;
+#pragma unroll 4
+  for (int c = ROW_J0 + wave; c < R; c += 4) {
+    const float jv = jok ? S1[(long)c * a.lds + jl] + S2j[(long)c * a.lds] : 0.f;
+    const float mu = wave_sum(aj * jv);
+    const float dj = jok ? jv - mu : 0.f;
+    if (jok) J[c * N + lane] = dj;
+    rpart += dj * dj;
   }
-  __syncthreads();
-  for (int j = tid; j < N; j += 256) {
-    float s = 0.f;
-    for (int c = ROW_J0; c < R; ++c) s += J[c * N + j] * J[c * N + j];
-    rj[j] = s;
-  }
+  rq[wave * 64 + lane] = rpart;
   __syncthreads();
   if (tid < 64) {
-    float nu = 0.f, lam = 0.f;
-    for (int j = tid; j < N; j += 64) {
-      nu += av[j] * rj[j];
-      lam += av[j] * lp[j];
-    }
-    nu = wave_sum(nu);
-    lam = wave_sum(lam);
+    const float r = rq[lane] + rq[64 + lane] + rq[128 + lane] + rq[192 + lane];
+    const float nu = wave_sum(jok ? av[lane] * r : 0.f);
+    const float lam = wave_sum(jok ? av[lane] * lp[lane] : 0.f);
+    rq[lane] = r;
     if (tid == 0) {
       red[0] = nu;
       red[1] = lam;
@@ -225,28 +230,31 @@ __global__ __launch_bounds__(256) void softmax_lap_kernel(SoftmaxArgs a) {
   }
   __syncthreads();
   const float nu = red[0], lam = red[1];
-  for (int idx = tid; idx < Rp * a.lda1; idx += 256) {
-    const int c = idx / a.lda1, j = idx % a.lda1;
-    float v = 0.f;
-    if (j < N) {
-      if (c == ROW_VAL)
-        v = av[j];
-      else if (c == ROW_LAP)
-        v = av[j] * (rj[j] - nu + lp[j] - lam);
-      else if (c < R)
-        v = av[j] * J[c * N + j];
+  // A1[c][j]: wave w writes rows c = w, w + 4, ...; lanes walk j (no index division)
+  for (int c = wave; c < Rp; c += 4) {
+    float* row = A1 + (long)c * a.lda1;
+    for (int j = lane; j < a.lda1; j += 64) {
+      float v = 0.f;
+      if (j < N) {
+        if (c == ROW_VAL)
+          v = av[j];
+        else if (c == ROW_LAP)
+          v = av[j] * (rq[j] - nu + lp[j] - lam);
+        else if (c < R)
+          v = av[j] * J[c * N + j];
+      }
+      row[j] = v;
     }
-    A1[idx] = v;
   }
   if (a.A2) {
+    // A2[(j, c)] = 2 a_j delta_jc, key-major; zero past N * Rp up to the 32-padding
     const long lda2 = ((long)N * Rp + 31) / 32 * 32;
     float* A2 = a.A2 + (long)z * N * lda2 + (long)i * lda2;
-    for (long idx = tid; idx < lda2; idx += 256) {
-      const int j = (int)(idx / Rp), c = (int)(idx % Rp);
-      float v = 0.f;
-      if (j < N && c >= ROW_J0 && c < R) v = 2.f * av[j] * J[c * N + j];
-      A2[idx] = v;
+    for (int j = wave; j < N; j += 4) {
+      const float two_a = 2.f * av[j];
+      for (int c = lane; c < Rp; c += 64) A2[(long)j * Rp + c] = (c >= ROW_J0 && c < R) ? two_a * J[c * N + j] : 0.f;
     }
+    for (long idx = (long)N * Rp + tid; idx < lda2; idx += 256) A2[idx] = 0.f;
   }
 }
 
@@ -271,8 +279,8 @@ int launch_layernorm_lap(int units, int Rp, int R, int D, const float* x, float*
 }
 
 int launch_softmax_lap(const SoftmaxArgs& a, hipStream_t s) {
-  const size_t smem = ((size_t)a.Rp * a.N + 3 * a.N + 8) * 4;
-  if (smem > 160 * 1024) return -1;
-  hipLaunchKernelGGL(softmax_lap_kernel, dim3(a.nz * a.N), dim3(256), smem, s, a);
+  const size_t smem = ((size_t)a.Rp * a.N + 2 * a.N + 256 + 8) * 4;
+  if (smem > 160 * 1024 || a.N > 64) return -1;
+  hipLaunchKernelGGL(softmax_lap_kernel, dim3((unsigned)(round_up(a.nz, 8) * a.N)), dim3(256), smem, s, a);
   return 0;
 }

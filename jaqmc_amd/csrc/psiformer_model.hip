@@ -184,7 +184,7 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
   P[1] = alloc(4 * nmax(uL * hmax, uF * hmax));                                 // T1
   P[2] = alloc(4 * nmax(uL * std::max(hmax, kin), uF * std::max(hmax, kin)));   // T2 / X0
   P[3] = alloc(4 * nmax(3 * uL * D, 3 * uF * D));                               // QKV (head-split)
-  P[4] = alloc(4 * nmax(uL * D, uF * D));                                       // attention output
+  P[4] = alloc(4 * nmax(uL * (D + 4), uF * (D + 4)));                           // attention output (padded rows)
   P[5] = alloc(4 * nmax(2L * H * uL * NpS + (long)H * N * NpS + (long)H * uL * Np32 + (long)H * N * lda2,
                         3L * H * uF * NpS + (long)H * uF * Np32));              // scores / softmax
   Phi = (float*)alloc(4 * nmax(uL * ldphi, uF * ldphi));
@@ -206,6 +206,10 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
   float* T2 = (float*)P[2];
   float* QKV = (float*)P[3];
   float* Om = (float*)P[4];
+  // Attention output rows are D + 4 floats apart: its Laplacian/Jacobian
+  // blocks are written as 256-byte segments one unit (Rp rows) apart, and a
+  // power-of-two unit stride would send them all to the same HBM channels.
+  const int DO = D + 4;
   const int NpS = round_up(N, 4), Np32 = round_up(N, 32);
   const long lda2 = round_up(N * RP, 32);
   float* S1 = (float*)P[5];
@@ -273,7 +277,20 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       rc |= launch_gemm(g, prec, false, s);
     }
     // ---- attention ----
-    {
+    if (lap && attn_lap_supported(N, dh)) {
+      ProfScope ps(PROF_ATTN_LAP, s);
+      AttnArgs aa{};
+      aa.QKV = QKV;
+      aa.O = Om;
+      aa.ldo = DO;
+      aa.N = N;
+      aa.Rp = RP;
+      aa.R = R;
+      aa.H = H;
+      aa.dh = dh;
+      aa.nz = Bc * H;
+      rc |= launch_attn_lap(aa, s);
+    } else {
       ProfScope ps(lap ? PROF_ATTN_LAP : -1, s);
       const long blk = (long)N * RP * dh;   // one (w, part, h) block of QKV
       const long sw = 3L * H * blk;
@@ -319,7 +336,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       BGemmArgs g4{};
       g4.X = A1; g4.ldx = Np32; g4.sxw = (long)H * N * RP * Np32; g4.sxh = (long)N * RP * Np32;
       g4.W = V; g4.ldw = (long)RP * dh; g4.sww = sw; g4.swh = blk;
-      g4.Y = Om; g4.ldy = D; g4.syw = (long)N * RP * D; g4.syh = dh;
+      g4.Y = Om; g4.ldy = DO; g4.syw = (long)N * RP * DO; g4.syh = dh;
       g4.rows = N * RP; g4.K = Np32; g4.ncols = dh; g4.kvalid = N; g4.nw = Bc; g4.nh = H; g4.nn = 1;
       rc |= launch_bgemm(g4, prec, s);
       if (lap) {
@@ -328,21 +345,21 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
         BGemmArgs g5{};
         g5.X = A1; g5.ldx = (long)RP * Np32; g5.sxw = g4.sxw; g5.sxh = g4.sxh;
         g5.W = V + dh; g5.ldw = (long)RP * dh; g5.sww = sw; g5.swh = blk;
-        g5.Y = Om + D; g5.ldy = (long)RP * D; g5.syw = g4.syw; g5.syh = dh; g5.Res = Om + D;
+        g5.Y = Om + DO; g5.ldy = (long)RP * DO; g5.syw = g4.syw; g5.syh = dh; g5.Res = Om + DO;
         g5.rows = N; g5.K = Np32; g5.ncols = (RP - 1) * dh; g5.kvalid = N; g5.nw = Bc; g5.nh = H; g5.nn = 1;
-        g5.ycw = dh; g5.ycs = D;
+        g5.ycw = dh; g5.ycs = DO;
         rc |= launch_bgemm(g5, prec, s);
         // O[i][1][d] += sum_{j,c} 2 J_a[i][(j,c)] V[(j,c)][d]
         BGemmArgs g6{};
         g6.X = A2; g6.ldx = lda2; g6.sxw = (long)H * N * lda2; g6.sxh = (long)N * lda2;
         g6.W = V; g6.ldw = dh; g6.sww = sw; g6.swh = blk;
-        g6.Y = Om + D; g6.ldy = (long)RP * D; g6.syw = g4.syw; g6.syh = dh; g6.Res = Om + D;
+        g6.Y = Om + DO; g6.ldy = (long)RP * DO; g6.syw = g4.syw; g6.syh = dh; g6.Res = Om + DO;
         g6.rows = N; g6.K = (int)lda2; g6.ncols = dh; g6.kvalid = N * RP; g6.nw = Bc; g6.nh = H; g6.nn = 1;
         rc |= launch_bgemm(g6, prec, s);
       }
     }
     // x += Wo O + bo
-    rc |= gemm(Om, D, ly.Wo, ly.Wo_lo, (long)D * D, D, x, D, D, ly.bo, ACT_NONE, x);
+    rc |= gemm(Om, DO, ly.Wo, ly.Wo_lo, (long)D * D, D, x, D, D, ly.bo, ACT_NONE, x);
     // MLP on LN2(x), residual into x
     rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln2_s, ly.ln2_b, 1e-5f, s);
     float* cur = T1;
