@@ -162,12 +162,18 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
   for (auto& ly : pl)
     for (int v : ly.mdims) hmax = std::max(hmax, round_up(v, 32));
   const long NpS = round_up(N, 4), Np32 = round_up(N, 32);
-  auto per_walker = [&](long R) {
+  // Laplacian-mode attention runs fused (no score tensors in HBM) when supported
+  const bool fused_attn = attn_lap_supported(N, dh);
+  auto attn_scratch = [&](long R) -> long {  // floats per walker for the score tensors
+    if (R > 1 && fused_attn) return 0;
     const long u = (long)N * R;
     const long lda2 = (long)round_up((int)(N * R), 32);
-    return (size_t)4 * (u * D + u * hmax + u * std::max<long>(hmax, kin) + 3 * u * D + u * D +
-                        2L * H * u * NpS + (long)H * N * NpS + (long)H * u * Np32 +
-                        (R > 1 ? (long)H * N * lda2 : 0) + u * round_up(K * N, 4) + (long)N * K * N * 5) +
+    return 2L * H * u * NpS + (long)H * N * NpS + (long)H * u * Np32 + (R > 1 ? (long)H * N * lda2 : 0);
+  };
+  auto per_walker = [&](long R) {
+    const long u = (long)N * R;
+    return (size_t)4 * (u * D + u * hmax + u * std::max<long>(hmax, kin) + 3 * u * D + u * (D + 4) +
+                        attn_scratch(R) + u * round_up(K * N, 4) + (long)N * K * N * 5) +
            (size_t)8 * K * (3 + T) + 64;
   };
   ldphi = round_up(K * N, 4);
@@ -179,14 +185,12 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
     return (size_t)std::max((long)chunk * per_lap, (long)chunk_fwd * per_fwd);
   };
   const long uL = (long)N * Rp, uF = N;
-  const long lda2 = round_up(N * Rp, 32);
   P[0] = alloc(4 * nmax(uL * D, uF * D));                                       // x
   P[1] = alloc(4 * nmax(uL * hmax, uF * hmax));                                 // T1
   P[2] = alloc(4 * nmax(uL * std::max(hmax, kin), uF * std::max(hmax, kin)));   // T2 / X0
   P[3] = alloc(4 * nmax(3 * uL * D, 3 * uF * D));                               // QKV (head-split)
   P[4] = alloc(4 * nmax(uL * (D + 4), uF * (D + 4)));                           // attention output (padded rows)
-  P[5] = alloc(4 * nmax(2L * H * uL * NpS + (long)H * N * NpS + (long)H * uL * Np32 + (long)H * N * lda2,
-                        3L * H * uF * NpS + (long)H * uF * Np32));              // scores / softmax
+  P[5] = alloc(4 * std::max<size_t>(1, nmax(attn_scratch(Rp), attn_scratch(1))));  // scores / softmax
   Phi = (float*)alloc(4 * nmax(uL * ldphi, uF * ldphi));
   env = (float*)alloc(4 * (size_t)std::max(chunk, chunk_fwd) * N * K * N * 5);
   det = (double*)alloc(8 * nmax((long)K * (3 + T), (long)K * 3));
