@@ -375,11 +375,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
   // accesses (16 rows of 64*NBW contiguous bytes).  The block loop is not
   // unrolled, so the store code exists once; the staging write selects the
   // block's accumulators with compile-time indices.
-  constexpr int WC = NBW * 16, SW = WC + 4, C4W = WC / 4;
+  constexpr int WC = NBW * 16, SW = WC + 4, C4W = WC / 4, RPI = 64 / C4W;
   static_assert(4 * 16 * SW * 4 <= ARENA, "staging slices must fit the arena");
+  static_assert(64 % C4W == 0 && 16 % RPI == 0, "store pattern");
   __syncthreads();  // every wave is done with the operand tiles
   float* stg = (float*)arena + wave * 16 * SW;
   const bool vec_io = (a.ldy % 4 == 0) && (!Res || a.ldres % 4 == 0) && (a.hs_H == 0 || a.hs_dh % 4 == 0);
+  // The lane's 4-column group is the same for every row it stores.
+  const int c4 = (lane % C4W) * 4, r_lane = lane / C4W;
+  const int n = col0 + wcol + c4;
+  const bool nok = n < a.ncols;
+  const bool tail = !(n + 4 <= a.ncols && vec_io);
+  // head-split destination: column part ((part*H + h)*N*RU)*dh + d is per lane,
+  // the row part ((w*3*H*N + i)*RU + rr)*dh per row; Laplacian-mode units of the
+  // tile are consecutive, so (w, i) of its first unit is computed once.
+  long hcol = 0;
+  int hw0 = 0, hi0 = 0;
+  if (a.hs_H > 0) {
+    const int part = n / a.hs_D, hh = (n % a.hs_D) / a.hs_dh, dd = n % a.hs_dh;
+    hcol = ((long)part * a.hs_H + hh) * a.hs_N * ((long)RU * a.hs_dh) + dd;
+    if (LAP) {
+      const int u0 = (int)(row0 / RU);
+      hw0 = u0 / a.hs_N;
+      hi0 = u0 % a.hs_N;
+    }
+  }
 #pragma unroll 1
   for (int mb0 = 0; mb0 < MB; ++mb0) {
 #pragma unroll
@@ -391,15 +411,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
           for (int v = 0; v < 4; ++v) stg[(4 * lg + v) * SW + nb * 16 + lr] = acc[mb][nb][v];
       }
     wave_sync();
-#pragma unroll 1
-    for (int idx = lane; idx < 16 * C4W; idx += 64) {
-      const int r = idx / C4W, c = (idx % C4W) * 4;
+#pragma unroll
+    for (int it = 0; it < 16 / RPI; ++it) {
+      const int r = r_lane + RPI * it;
       const long grow = row0 + mb0 * 16 + r;
-      const int n = col0 + wcol + c;
-      if (grow >= a.rows || n >= a.ncols) continue;
+      if (grow >= a.rows || !nok) continue;
       float y[4];
-      *(float4*)y = *(const float4*)&stg[r * SW + c];
-      const bool tail = !(n + 4 <= a.ncols && vec_io);
+      *(float4*)y = *(const float4*)&stg[r * SW + c4];
       if (!LAP) {
         // plain mode: broadcast term, bias and activation
         const float* gp = nullptr;
@@ -418,12 +436,22 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
       }
       float* dst;
       if (a.hs_H > 0) {
-        // destination [w][part][head][i][RU][dh] of walker-major unit w*hs_N + i
-        const int unit = (int)(LAP ? grow / RU : grow);
-        const int rr = LAP ? (int)(grow % RU) : 0;
-        const int w = unit / a.hs_N, i = unit % a.hs_N;
-        const int part = n / a.hs_D, hh = (n % a.hs_D) / a.hs_dh, dd = n % a.hs_dh;
-        dst = Y + ((((long)w * 3 + part) * a.hs_H + hh) * a.hs_N + i) * ((long)RU * a.hs_dh) + (long)rr * a.hs_dh + dd;
+        int w, i, rr;
+        if (LAP) {
+          const int u = (mb0 * 16 + r) / RU;  // unit of the tile
+          rr = (mb0 * 16 + r) % RU;
+          i = hi0 + u;
+          w = hw0;
+          while (i >= a.hs_N) {
+            i -= a.hs_N;
+            ++w;
+          }
+        } else {
+          w = (int)(grow / a.hs_N);
+          i = (int)(grow % a.hs_N);
+          rr = 0;
+        }
+        dst = Y + ((((long)w * 3 * a.hs_H) * a.hs_N + i) * RU + rr) * (long)a.hs_dh + hcol;
       } else {
         dst = Y + grow * a.ldy + n;
         if (Res) {
