@@ -10,6 +10,8 @@
 //   * means over electrons are linear, so they are expanded to dense rows when
 //     aggregated (reductions.py:25-61).
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace {
@@ -20,35 +22,51 @@ namespace {
 //   ee[a, b, 0]  = |x_a - x_b| (0 on the diagonal), ee[a, b, 1 + d] = (x_a - x_b)_d
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void ae_features_kernel(FermiDims d, const float* __restrict__ elec, const float* __restrict__ atoms,
-                                   T* __restrict__ X, long ldx) {
-  const int F = 4 * d.M;
-  const long total = (long)d.Bc * d.N * d.Rp * F;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    const int f = idx % F;
-    const long ur = idx / F;
-    const int r = ur % d.Rp;
-    const int u = ur / d.Rp;
-    const int w = walker_of_unit(u, d.Bc, d.n_up, d.n_dn);
-    const int i = elec_of_unit(u, d.Bc, d.n_up, d.n_dn);
-    const int I = f >> 2, e = f & 3;
-    const float* xi = elec + ((long)w * d.N + i) * 3;
+__global__ __launch_bounds__(256) void ae_features_kernel(FermiDims d, const float* __restrict__ elec,
+                                                          const float* __restrict__ atoms, T* __restrict__ X, long ldx) {
+  // One workgroup per unit (electron of a walker): the displacements to the M
+  // nuclei are computed once into LDS; thread items are (row r, nucleus I),
+  // each writing the four features (|d|, d) of one nucleus.
+  extern __shared__ float geo[];  // [M][4]: dx, dy, dz, r
+  const int u = blockIdx.x;
+  const int w = walker_of_unit(u, d.Bc, d.n_up, d.n_dn), i = elec_of_unit(u, d.Bc, d.n_up, d.n_dn);
+  const float* xi = elec + ((long)w * d.N + i) * 3;
+  for (int I = threadIdx.x; I < d.M; I += blockDim.x) {
     const float dx = xi[0] - atoms[3 * I], dy = xi[1] - atoms[3 * I + 1], dz = xi[2] - atoms[3 * I + 2];
-    const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
-    const float dv[3] = {dx, dy, dz};
-    float out = 0.f;
+    geo[4 * I] = dx;
+    geo[4 * I + 1] = dy;
+    geo[4 * I + 2] = dz;
+    geo[4 * I + 3] = sqrtf(dx * dx + dy * dy + dz * dz);
+  }
+  __syncthreads();
+  T* Xu = X + (long)u * d.Rp * ldx;
+  const int items = d.Rp * d.M;
+  for (int idx = threadIdx.x; idx < items; idx += blockDim.x) {
+    const int r = idx / d.M, I = idx - r * d.M;
+    const float4 g = *(const float4*)&geo[4 * I];
+    float o[4] = {0.f, 0.f, 0.f, 0.f};
     if (r == ROW_VAL) {
-      out = e == 0 ? rr : dv[e - 1];
+      o[0] = g.w;
+      o[1] = g.x;
+      o[2] = g.y;
+      o[3] = g.z;
     } else if (r == ROW_LAP) {
-      out = e == 0 ? 2.f / rr : 0.f;
+      o[0] = 2.f / g.w;
     } else {
       const int c = r - ROW_J0;
       if (c < 3 * d.N && c / 3 == i) {
-        const int ax = c % 3;
-        out = e == 0 ? dv[ax] / rr : (ax == e - 1 ? 1.f : 0.f);
+        const int ax = c - 3 * i;
+        o[0] = (ax == 0 ? g.x : (ax == 1 ? g.y : g.z)) / g.w;
+        o[1 + ax] = 1.f;
       }
     }
-    X[(long)u * d.Rp * ldx + (long)r * ldx + f] = from_f<T>(out);
+    T* dst = Xu + (long)r * ldx + 4 * I;
+    if constexpr (std::is_same<T, float>::value) {
+      *(float4*)dst = *(float4*)o;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) dst[e] = from_f<T>(o[e]);
+    }
   }
 }
 
@@ -321,45 +339,55 @@ __global__ void spin_means_kernel(FermiDims d, const T* __restrict__ X, long ldx
 // channel s, det k, orbital o:  e = sum_I pi[o,I,k] exp(-a r_iI), a = |sigma| or
 // sigma.  Gradient wrt x_i: sum_I -a f (x_i - R_I)/r; Laplacian: sum_I f (a^2 - 2a/r).
 // Output env[w][i][k*N + o][5] = (value, Laplacian, d/dx, d/dy, d/dz).
-__global__ void envelope_kernel(FermiDims d, const float* __restrict__ elec, const float* __restrict__ atoms,
-                                const float* __restrict__ pi0, const float* __restrict__ sg0,
-                                const float* __restrict__ pi1, const float* __restrict__ sg1, int kind,
-                                float* __restrict__ env) {
-  const long total = (long)d.Bc * d.N * d.K * d.N;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    const int o = idx % d.N;
-    const int k = (idx / d.N) % d.K;
-    const int i = (idx / ((long)d.N * d.K)) % d.N;
-    const int w = idx / ((long)d.N * d.K * d.N);
-    float* out = env + idx * 5;
-    if (kind == 2) {  // null envelope
-      out[0] = 1.f;
-      out[1] = out[2] = out[3] = out[4] = 0.f;
-      continue;
+// One workgroup per (walker, electron): the electron-nucleus displacements are
+// computed once into LDS, and the threads walk (k, o) with the parameters laid
+// out [I][k][o] (coalesced across the threads for each nucleus).
+__global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float* __restrict__ elec,
+                                                       const float* __restrict__ atoms, const float* __restrict__ pi0,
+                                                       const float* __restrict__ sg0, const float* __restrict__ pi1,
+                                                       const float* __restrict__ sg1, int kind,
+                                                       float* __restrict__ env) {
+  extern __shared__ float geo[];  // [M][4]: dx, dy, dz, r
+  const int wi = blockIdx.x, i = wi % d.N;
+  const int KN = d.K * d.N;
+  float* out = env + (long)wi * KN * 5;
+  if (kind == 2) {  // null envelope
+    for (int ko = threadIdx.x; ko < KN; ko += blockDim.x) {
+      out[ko * 5] = 1.f;
+      out[ko * 5 + 1] = out[ko * 5 + 2] = out[ko * 5 + 3] = out[ko * 5 + 4] = 0.f;
     }
-    const bool up = d.nch == 1 || i < d.n_up;
-    const float* pi = up ? pi0 : pi1;
-    const float* sg = up ? sg0 : sg1;
-    const float* xi = elec + ((long)w * d.N + i) * 3;
+    return;
+  }
+  const float* xi = elec + (long)wi * 3;
+  for (int I = threadIdx.x; I < d.M; I += blockDim.x) {
+    const float dx = xi[0] - atoms[3 * I], dy = xi[1] - atoms[3 * I + 1], dz = xi[2] - atoms[3 * I + 2];
+    geo[4 * I] = dx;
+    geo[4 * I + 1] = dy;
+    geo[4 * I + 2] = dz;
+    geo[4 * I + 3] = sqrtf(dx * dx + dy * dy + dz * dz);
+  }
+  __syncthreads();
+  const bool up = d.nch == 1 || i < d.n_up;
+  const float* pi = up ? pi0 : pi1;
+  const float* sg = up ? sg0 : sg1;
+  for (int ko = threadIdx.x; ko < KN; ko += blockDim.x) {
     float v = 0.f, l = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     for (int I = 0; I < d.M; ++I) {
-      const float dx = xi[0] - atoms[3 * I], dy = xi[1] - atoms[3 * I + 1], dz = xi[2] - atoms[3 * I + 2];
-      const float rr = sqrtf(dx * dx + dy * dy + dz * dz);
-      const long pidx = ((long)o * d.M + I) * d.K + k;
-      const float a = kind == 1 ? fabsf(sg[pidx]) : sg[pidx];
-      const float f = pi[pidx] * expf(-a * rr);
+      const float4 g = *(const float4*)&geo[4 * I];
+      const float a = kind == 1 ? fabsf(sg[(long)I * KN + ko]) : sg[(long)I * KN + ko];
+      const float f = pi[(long)I * KN + ko] * expf(-a * g.w);
       v += f;
-      l += f * (a * a - 2.f * a / rr);
-      const float gs = -a * f / rr;
-      gx += gs * dx;
-      gy += gs * dy;
-      gz += gs * dz;
+      l += f * (a * a - 2.f * a / g.w);
+      const float gs = -a * f / g.w;
+      gx += gs * g.x;
+      gy += gs * g.y;
+      gz += gs * g.z;
     }
-    out[0] = v;
-    out[1] = l;
-    out[2] = gx;
-    out[3] = gy;
-    out[4] = gz;
+    out[ko * 5] = v;
+    out[ko * 5 + 1] = l;
+    out[ko * 5 + 2] = gx;
+    out[ko * 5 + 3] = gy;
+    out[ko * 5 + 4] = gz;
   }
 }
 
@@ -373,11 +401,14 @@ inline int grid_for(long total) {
 
 int launch_ferminet_features(const FermiDims& d, const float* elec, const float* atoms, void* X, long ldx,
                              bool x_bf16, float* H2, hipStream_t s) {
-  const long t1 = (long)d.Bc * d.N * d.Rp * 4 * d.M;
-  if (x_bf16)
-    hipLaunchKernelGGL(ae_features_kernel<bf16>, grid_for(t1), 256, 0, s, d, elec, atoms, (bf16*)X, ldx);
-  else
-    hipLaunchKernelGGL(ae_features_kernel<float>, grid_for(t1), 256, 0, s, d, elec, atoms, (float*)X, ldx);
+  const unsigned units = (unsigned)(d.Bc * d.N);
+  const size_t smem = (size_t)4 * d.M * sizeof(float);
+  if (units > 0 && smem <= 64 * 1024) {
+    if (x_bf16)
+      hipLaunchKernelGGL(ae_features_kernel<bf16>, dim3(units), 256, smem, s, d, elec, atoms, (bf16*)X, ldx);
+    else
+      hipLaunchKernelGGL(ae_features_kernel<float>, dim3(units), 256, smem, s, d, elec, atoms, (float*)X, ldx);
+  }
   const long t2 = (long)d.Bc * d.N * d.N;
   hipLaunchKernelGGL(ee_features_kernel, grid_for(t2), 256, 0, s, d, d.Rp > 1 ? 8 : 1, elec, H2);
   return 0;
@@ -450,8 +481,10 @@ int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void*
 
 int launch_envelope(const FermiDims& d, const float* elec, const float* atoms, const float* const* pi,
                     const float* const* sigma, int env_kind, float* env, hipStream_t s) {
-  const long total = (long)d.Bc * d.N * d.K * d.N;
-  hipLaunchKernelGGL(envelope_kernel, grid_for(total), 256, 0, s, d, elec, atoms, pi[0], sigma[0],
+  if ((long)d.Bc * d.N <= 0) return 0;
+  const size_t smem = (size_t)4 * d.M * sizeof(float);
+  if (smem > 64 * 1024) return -1;
+  hipLaunchKernelGGL(envelope_kernel, dim3((unsigned)(d.Bc * d.N)), 256, smem, s, d, elec, atoms, pi[0], sigma[0],
                      d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
   return 0;
 }

@@ -10,6 +10,17 @@
 int jq_fail(int code, const std::string& msg);
 
 // One FermiNet layer (FermiLayers, wavefunction/backbone/ferminet.py:31-63).
+// Envelope parameters arrive as (orbital o, atom I, det k) (the Flax
+// (N_orb, M, K) layout); the device keeps them as [I][k][o] so that the
+// threads of an envelope workgroup, which walk (k, o), read them coalesced.
+inline std::vector<float> envelope_device_layout(const float* p, int N, int M, int K) {
+  std::vector<float> t((size_t)N * M * K);
+  for (int o = 0; o < N; ++o)
+    for (int I = 0; I < M; ++I)
+      for (int k = 0; k < K; ++k) t[((size_t)I * K + k) * N + o] = p[((size_t)o * M + I) * K + k];
+  return t;
+}
+
 struct FermiLayer {
   int f1, f2;        // one- / two-electron input widths
   int d1, d2;        // output widths

@@ -87,9 +87,11 @@ static int read_outputs(jq_model* m, Reader& rd, int d_last) {
     const float* pi = rd.take(ne);
     const float* sg = rd.take(ne);
     if (!pi || !sg) return jq_fail(JQ_EINVAL, "parameter pack too short (envelope)");
-    int rc = m->upload(pi, ne * 4, (void**)&m->pi[c]);
+    const std::vector<float> pt = envelope_device_layout(pi, m->N, m->M, m->K);
+    const std::vector<float> st = envelope_device_layout(sg, m->N, m->M, m->K);
+    int rc = m->upload(pt.data(), ne * 4, (void**)&m->pi[c]);
     if (rc) return rc;
-    rc = m->upload(sg, ne * 4, (void**)&m->sigma[c]);
+    rc = m->upload(st.data(), ne * 4, (void**)&m->sigma[c]);
     if (rc) return rc;
   }
   return 0;

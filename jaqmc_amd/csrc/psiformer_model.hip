@@ -146,8 +146,10 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
       const float* pp = rd.take(ne);
       const float* sg = rd.take(ne);
       if (!pp || !sg) return jq_fail(JQ_EINVAL, "parameter pack too short (envelope)");
-      int rc = upload(pp, ne * 4, (void**)&pi[c]);
-      if (!rc) rc = upload(sg, ne * 4, (void**)&sigma[c]);
+      const std::vector<float> pt = envelope_device_layout(pp, N, M, K);
+      const std::vector<float> st = envelope_device_layout(sg, N, M, K);
+      int rc = upload(pt.data(), ne * 4, (void**)&pi[c]);
+      if (!rc) rc = upload(st.data(), ne * 4, (void**)&sigma[c]);
       if (rc) return rc;
     }
   }
