@@ -249,25 +249,29 @@ __global__ __launch_bounds__(256) void pair_means_kernel(FermiDims d, int RP2, c
   const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
   const int ncol = (int)(ldx - col0);
   T* xu = X + (long)u * d.Rp * ldx + col0;
-  for (int idx = tid; idx < d.Rp * ncol; idx += 256) {
-    const int r = idx / ncol, cc = idx % ncol;
-    float out = 0.f;
-    if (cc < d.nch * f2) {
-      const int s = cc / f2, f = cc % f2;
-      const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
-      const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
-      if (r == ROW_VAL || r == ROW_LAP) {
-        out = part[(s * 5 + r) * f2 + f];
-      } else if (r - ROW_J0 < 3 * d.N) {
-        const int c = r - ROW_J0;
-        const int e = c / 3, ax = c % 3;
-        if (e == i)
-          out = part[(s * 5 + 2 + ax) * f2 + f];
-        else if (e >= j0 && e < j0 + ns)
-          out = col[(e * RP2 + 2 + ax) * f2 + f] / ns;
+  // lanes walk columns (per-column constants hoisted), the four waves walk rows
+  for (int cc = tid & 63; cc < ncol; cc += 64) {
+    const bool live = cc < d.nch * f2;
+    const int s = live ? cc / f2 : 0, f = live ? cc - s * f2 : 0;
+    const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
+    const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
+    const float inv = 1.f / ns;
+    for (int r = tid >> 6; r < d.Rp; r += 4) {
+      float out = 0.f;
+      if (live) {
+        if (r == ROW_VAL || r == ROW_LAP) {
+          out = part[(s * 5 + r) * f2 + f];
+        } else if (r - ROW_J0 < 3 * d.N) {
+          const int c = r - ROW_J0;
+          const int e = c / 3, ax = c - 3 * e;
+          if (e == i)
+            out = part[(s * 5 + 2 + ax) * f2 + f];
+          else if (e >= j0 && e < j0 + ns)
+            out = col[(e * RP2 + 2 + ax) * f2 + f] * inv;
+        }
       }
+      xu[(long)r * ldx + cc] = from_f<T>(out);
     }
-    xu[(long)r * ldx + cc] = from_f<T>(out);
   }
 }
 
@@ -286,6 +290,7 @@ __global__ __launch_bounds__(256) void spin_means4_kernel(FermiDims d, const flo
   for (int idx = threadIdx.x; idx < d.Rp * q4; idx += 256) {
     const int r = idx / q4, f = (idx % q4) * 4;
     float4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
     for (int k = 0; k < ns; ++k) {
       const float4 v = *(const float4*)(X + ((long)(u0 + k) * d.Rp + r) * ldx + f);
       acc.x += v.x;
