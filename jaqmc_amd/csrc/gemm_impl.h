@@ -66,8 +66,15 @@ inline int xcd_grid(long nrt, int ncb) { return (int)((nrt + kXcds - 1) / kXcds)
 
 // MBU: 16-row blocks per unit (Laplacian mode); UPT: units per tile; NBW:
 // 16-column blocks per wave.  BM = 16*MBU*UPT, BN = 64*NBW.
+// Waves per SIMD the register allocation must leave room for: three for tiles
+// of at most 12 accumulator blocks, two otherwise.
+template <int MB, int NBW>
+constexpr int gemm_min_waves() {
+  return MB * NBW <= 12 ? 3 : 2;
+}
+
 template <int MBU, int UPT, int NBW, int PREC, bool LAP>
-__global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
+__global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_kernel(const GemmArgs a) {
   constexpr int MB = MBU * UPT;
   constexpr int BM = 16 * MB;
   constexpr int BN = 64 * NBW;
@@ -77,18 +84,22 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
   constexpr int LS = F32 ? 36 : 40;  // LDS row stride (elements)
   using TL = typename std::conditional<F32, float, bf16>::type;
   constexpr int NSL = F32 ? 1 : NS;
-  // LDS holds the operand tiles of one K chunk (the epilogue runs in registers)
-  constexpr int ARENA = NSL * (BM + BN) * LS * (int)sizeof(TL);
+  // LDS holds the activation tile of one K chunk, which all four waves read;
+  // the epilogue reuses it to stage output rows.  The weights are not staged:
+  // every wave owns its own columns, so each lane loads its MFMA B fragments
+  // (8 consecutive k of one column) straight from global memory, where the
+  // weights stay L2-resident, one K chunk ahead.
+  constexpr int A_BYTES = NSL * BM * LS * (int)sizeof(TL);
+  constexpr int STG_BYTES = 4 * 16 * (NBW * 16 + 4) * 4;
+  constexpr int ARENA = A_BYTES > STG_BYTES ? A_BYTES : STG_BYTES;
   __shared__ __attribute__((aligned(16))) char arena[ARENA];
   TL (*As)[BM * LS] = (TL (*)[BM * LS])arena;
-  TL (*Bs)[BN * LS] = (TL (*)[BN * LS])(arena + NSL * BM * LS * sizeof(TL));
 
-  // register prefetch: A and B chunks as float4 (8 per 32-wide row)
+  // register prefetch: A chunk as float4 (8 per 32-wide row); B fragments
   constexpr int AV = (BM * 8 + 255) / 256;
-  constexpr int BV = F32 ? (BN * 8 + 255) / 256 : 1;
-  constexpr int BVB = F32 ? 1 : (BN * 4 + 255) / 256;
-  float4 pa[AV], pb[BV];
-  uint4 pbb[NS][BVB];
+  float4 pa[AV];
+  float4 bn32[NBW][2];  // FP32: k = 8g .. 8g+7 of column nb
+  uint4 bnb[NS][NBW];   // BF16 planes: the same 8 k as bf16x8
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -107,8 +118,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
   const int col0 = cb * BN;
   const float* X = (const float*)a.X + (long)blockIdx.z * a.sxb;
   const float* W = (const float*)a.Wt;
+  const bf16* Wb = (const bf16*)a.Wt_lo;
 
-  auto gload = [&](int k0) {
+  auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int v = 0; v < AV; ++v) {
       const int idx = tid + v * 256;
@@ -117,31 +129,24 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
       pa[v] = (idx < BM * 8 && gr < a.rows) ? *(const float4*)(X + gr * a.ldx + k0 + q * 4)
                                              : float4{0.f, 0.f, 0.f, 0.f};
     }
-    if constexpr (F32) {
+  };
+  auto bload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
-      for (int v = 0; v < BV; ++v) {
-        const int idx = tid + v * 256;
-        const int c = idx >> 3, q = idx & 7;
-        const int gc = col0 + c;
-        pb[v] = (idx < BN * 8 && gc < a.ncols) ? *(const float4*)(W + (long)gc * a.ldw + k0 + q * 4)
-                                               : float4{0.f, 0.f, 0.f, 0.f};
+    for (int nb = 0; nb < NBW; ++nb) {
+      const int gc = col0 + wcol + nb * 16 + lr;
+      const bool ok = gc < a.ncols;
+      if constexpr (F32) {
+        const float* p = W + (long)(ok ? gc : 0) * a.ldw + k0 + 8 * lg;
+        bn32[nb][0] = ok ? *(const float4*)p : float4{0.f, 0.f, 0.f, 0.f};
+        bn32[nb][1] = ok ? *(const float4*)(p + 4) : float4{0.f, 0.f, 0.f, 0.f};
+      } else {
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          bnb[s][nb] = ok ? *(const uint4*)(Wb + s * a.wplane + (long)gc * a.ldw + k0 + 8 * lg) : uint4{0u, 0u, 0u, 0u};
       }
-    } else {
-      const bf16* Wb = (const bf16*)a.Wt_lo;
-#pragma unroll
-      for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int v = 0; v < BVB; ++v) {
-          const int idx = tid + v * 256;
-          const int c = idx >> 2, q = idx & 3;
-          const int gc = col0 + c;
-          pbb[s][v] = (idx < BN * 4 && gc < a.ncols)
-                          ? *(const uint4*)(Wb + s * a.wplane + (long)gc * a.ldw + k0 + q * 8)
-                          : uint4{0u, 0u, 0u, 0u};
-        }
     }
   };
-  auto lstore = [&]() {
+  auto lstore = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int v = 0; v < AV; ++v) {
       const int idx = tid + v * 256;
@@ -161,25 +166,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
           *(uint2*)(&As[s][r * LS + q * 4]) = *(uint2*)h;
         }
       }
-    }
-    if constexpr (F32) {
-#pragma unroll
-      for (int v = 0; v < BV; ++v) {
-        const int idx = tid + v * 256;
-        if (idx >= BN * 8) break;
-        const int c = idx >> 3, q = idx & 7;
-        *(float4*)(&Bs[0][c * LS + q * 4]) = pb[v];
-      }
-    } else {
-#pragma unroll
-      for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int v = 0; v < BVB; ++v) {
-          const int idx = tid + v * 256;
-          if (idx >= BN * 4) break;
-          const int c = idx >> 2, q = idx & 3;
-          *(uint4*)(&Bs[s][c * LS + q * 8]) = pbb[s][v];
-        }
     }
   };
 
@@ -231,19 +217,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
   }
 
   gload(0);
+  bload(0);
   for (int k0 = 0; k0 < a.K; k0 += KC) {
     lstore();
-    __syncthreads();
-    if (k0 + KC < a.K) gload(k0 + KC);
-
+    // this chunk's B fragments; the next chunk's are loaded meanwhile
+    float bf[NBW][8];
+    bf16x8 bt[NS][NBW];
     if constexpr (F32) {
-      float bf[NBW][8];
 #pragma unroll
       for (int nb = 0; nb < NBW; ++nb) {
-        const float* p = &Bs[0][(wcol + nb * 16 + lr) * LS + lg * 8];
-        *(float4*)&bf[nb][0] = *(const float4*)p;
-        *(float4*)&bf[nb][4] = *(const float4*)(p + 4);
+        *(float4*)&bf[nb][0] = bn32[nb][0];
+        *(float4*)&bf[nb][4] = bn32[nb][1];
       }
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) bt[s][nb] = __builtin_bit_cast(bf16x8, bnb[s][nb]);
+    }
+    __syncthreads();
+    if (k0 + KC < a.K) {
+      gload(k0 + KC);
+      bload(k0 + KC);
+    }
+
+    if constexpr (F32) {
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
         float af[8];
@@ -257,11 +255,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmArgs a) {
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kk], bf[nb][kk], acc[mb][nb], 0, 0, 0);
       }
     } else {
-      bf16x8 bt[NS][NBW];
-#pragma unroll
-      for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int nb = 0; nb < NBW; ++nb) bt[s][nb] = *(const bf16x8*)&Bs[s][(wcol + nb * 16 + lr) * LS + lg * 8];
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
         bf16x8 at[NS];
