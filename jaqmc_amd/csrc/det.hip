@@ -485,6 +485,82 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
 }
 
 // ---------------------------------------------------------------------------
+// Forward mode (log|det| and sign only), 16 < N <= 64: one wavefront per
+// determinant, two per workgroup, no workgroup barriers.  LU with partial
+// pivoting on the N x N block in the wave's LDS slice (no inverse is formed):
+// lane (ri, cj) = (lane / 8, lane % 8) updates rows p+1+ri+8a, columns
+// p+1+cj+8b of the trailing block, with its pivot-row entries in registers.
+// ---------------------------------------------------------------------------
+template <int NP>
+__global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+                                                      const float* __restrict__ env, double* __restrict__ out,
+                                                      long ndet) {
+  constexpr int S = NP + 1;  // odd row stride (doubles)
+  constexpr int NB8 = NP / 8;
+  extern __shared__ double smf[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long det = (long)blockIdx.x * 2 + wv;
+  if (det >= ndet) return;  // the wave's own determinant; nothing is shared across waves
+  double* M = smf + wv * NP * S;
+  const int N = d.N;
+  const int w = (int)(det / d.K), k = (int)(det % d.K);
+  for (int e = lane; e < N * N; e += 64) {
+    const int i = e / N, o = e - i * N;
+    M[i * S + o] = (double)Phi[(long)unit_idx(d, w, i) * d.Rp * ldphi + (long)k * N + o] *
+                   (double)env_ptr(d, env, w, i, k, o)[0];
+  }
+  wave_sync();
+  double logdet = 0.0;
+  int sgn = 1;
+  const int ri = lane >> 3, cj = lane & 7;
+  for (int p = 0; p < N; ++p) {
+    double best = (p + lane < N) ? fabs(M[(p + lane) * S + p]) : -1.0;
+    int bi = p + lane;
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o);
+      const int oi = __shfl_xor(bi, o);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (bi != p) {
+      if (lane < N) {
+        const double t = M[p * S + lane];
+        M[p * S + lane] = M[bi * S + lane];
+        M[bi * S + lane] = t;
+      }
+      sgn = -sgn;
+    }
+    wave_sync();
+    const double pv = M[p * S + p];
+    logdet += log(fabs(pv));
+    if (pv < 0) sgn = -sgn;
+    const double rp = 1.0 / pv;
+    double prow[NB8];
+#pragma unroll
+    for (int b = 0; b < NB8; ++b) {
+      const int j = p + 1 + cj + 8 * b;
+      prow[b] = j < N ? M[p * S + j] : 0.0;
+    }
+    for (int i = p + 1 + ri; i < N; i += 8) {
+      const double l = M[i * S + p] * rp;
+#pragma unroll
+      for (int b = 0; b < NB8; ++b) {
+        const int j = p + 1 + cj + 8 * b;
+        if (j < N) M[i * S + j] -= l * prow[b];
+      }
+    }
+    wave_sync();
+  }
+  if (lane == 0) {
+    out[det * 3] = logdet;
+    out[det * 3 + 1] = (double)sgn;
+    out[det * 3 + 2] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Small systems (N <= 16): one wavefront per determinant, four per workgroup,
 // no workgroup barriers.  The augmented [A | I] (16 x 32 FP64) is reduced by
 // Gauss-Jordan with partial pivoting in the wave's LDS slice; then lane
@@ -842,6 +918,22 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
       case 32: hipLaunchKernelGGL(det_mfma_kernel<32>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
       case 48: hipLaunchKernelGGL(det_mfma_kernel<48>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
       case 64: hipLaunchKernelGGL(det_mfma_kernel<64>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
+    }
+  }
+  if (d.Rp <= 1 && d.N <= 64) {
+    const long ndet = (long)d.Bc * d.K;
+    const int np = round_up(d.N, 16);
+    const size_t smem = (size_t)2 * np * (np + 1) * sizeof(double);
+    static bool attr_f = false;
+    if (!attr_f) {
+      (void)hipFuncSetAttribute((const void*)det_fwd_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024);
+      attr_f = true;
+    }
+    const dim3 grid((unsigned)((ndet + 1) / 2));
+    switch (np) {
+      case 32: hipLaunchKernelGGL(det_fwd_kernel<32>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
+      case 48: hipLaunchKernelGGL(det_fwd_kernel<48>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
+      case 64: hipLaunchKernelGGL(det_fwd_kernel<64>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
     }
   }
   int Np, cpi;
