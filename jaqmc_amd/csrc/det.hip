@@ -302,6 +302,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
   const long rowstride = (long)d.Rp * ldphi;  // floats per unit
   const long kN = (long)k * N;
 
+#pragma unroll 4
   for (int e = tid; e < NP * NP; e += 256) {
     const int i = e / NP, o = e % NP;
     double a = (i == o) ? 1.0 : 0.0;
@@ -379,6 +380,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
 
   // tr(A^-1 L), L = lap(phi env) = lap(phi) env + phi lap(env) + 2 grad(phi).grad(env)
   double part = 0.0;
+#pragma unroll 4
   for (int e = tid; e < N * N; e += 256) {
     const int i = e / N, o = e % N;
     const float* ph = Phi + (long)unit_idx(d, w, i) * rowstride + kN + o;
@@ -504,6 +506,7 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
   double* M = smf + wv * NP * S;
   const int N = d.N;
   const int w = (int)(det / d.K), k = (int)(det % d.K);
+#pragma unroll 8
   for (int e = lane; e < N * N; e += 64) {
     const int i = e / N, o = e - i * N;
     M[i * S + o] = (double)Phi[(long)unit_idx(d, w, i) * d.Rp * ldphi + (long)k * N + o] *

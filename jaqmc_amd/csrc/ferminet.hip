@@ -230,9 +230,20 @@ __global__ __launch_bounds__(256) void pair_means_kernel(FermiDims d, int RP2, c
   const int tid = threadIdx.x;
   const long pstride = (long)RP2 * f2;
   const float* base = H2 + ((long)w * d.N * d.N + i) * pstride;  // pair (0, i)
-  for (int idx = tid; idx < d.N * RP2 * f2; idx += 256) {
-    const int j = idx / (RP2 * f2), q = idx % (RP2 * f2);
-    col[idx] = base[(long)j * d.N * pstride + q];
+  if ((RP2 * f2) % 4 == 0) {
+    // 16-byte loads, eight in flight per thread
+    const int q4 = RP2 * f2 / 4;
+#pragma unroll 8
+    for (int idx = tid; idx < d.N * q4; idx += 256) {
+      const int j = idx / q4, q = idx - j * q4;
+      *(float4*)&col[4 * idx] = *(const float4*)(base + (long)j * d.N * pstride + 4 * q);
+    }
+  } else {
+#pragma unroll 8
+    for (int idx = tid; idx < d.N * RP2 * f2; idx += 256) {
+      const int j = idx / (RP2 * f2), q = idx % (RP2 * f2);
+      col[idx] = base[(long)j * d.N * pstride + q];
+    }
   }
   __syncthreads();
   const int nrow_sum = RP2 > 1 ? 5 : 1;
