@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
   float m = 0.f;
   if (tid < Rp) {
     const float4* row = (const float4*)(xu + (long)tid * D);
-#pragma unroll 4
+#pragma unroll 8
     for (int q = 0; q < D / 4; ++q) {
       const float4 v = row[q];
       m += (v.x + v.y) + (v.z + v.w);
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
   float dot = 0.f, sq = 0.f;
   if (tid < Rp) {
     const float4* row = (const float4*)(xu + (long)tid * D);
-#pragma unroll 4
+#pragma unroll 8
     for (int q = 0; q < D / 4; ++q) {
       const float4 v = row[q];
       const float a0 = v.x - m, a1 = v.y - m, a2 = v.z - m, a3 = v.w - m;
@@ -139,6 +139,7 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
   for (int f = tid; f < D; f += 256) {
     const float g = scale[f], d0 = d0s[f];
     float cross = 0.f;
+#pragma unroll 8
     for (int r = ROW_J0; r < R; ++r) {
       const float dr = xu[(long)r * D + f] - mu[r];
       cross += dr * js[r];
@@ -148,6 +149,93 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
     yu[f] = g * d0 * sv + bias[f];
     if (Rp > 1) {
       const float dl = xu[D + f] - mu[1];
+      yu[D + f] = g * (dl * sv + d0 * laps + 2.f * cross);
+    }
+  }
+}
+
+// The same rule with the unit's Rp x D tile read from HBM once into LDS (row
+// stride D + 1 floats: the row-per-thread statistics passes and the
+// feature-per-thread output pass are both bank-conflict free), used when the
+// tile fits.
+template <int D>
+__global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, const float* __restrict__ x,
+                                                                float* __restrict__ y, const float* __restrict__ scale,
+                                                                const float* __restrict__ bias, float eps) {
+  constexpr int SD = D + 1;
+  extern __shared__ float tile[];  // Rp x SD
+  __shared__ float d0s[D];
+  __shared__ float mu[256], js[256], red[8];
+  const long base = (long)blockIdx.x * Rp * D;
+  const int tid = threadIdx.x;
+  const float4* x4 = (const float4*)(x + base);
+  constexpr int Q4 = D / 4;
+#pragma unroll 8
+  for (int idx = tid; idx < Rp * Q4; idx += 256) {
+    const int r = idx / Q4, q = idx - r * Q4;
+    const float4 v = x4[idx];
+    float* t = tile + r * SD + 4 * q;
+    t[0] = v.x;
+    t[1] = v.y;
+    t[2] = v.z;
+    t[3] = v.w;
+  }
+  __syncthreads();
+  float m = 0.f;
+  if (tid < Rp) {
+    const float* row = tile + tid * SD;
+#pragma unroll 8
+    for (int f = 0; f < D; ++f) m += row[f];
+    m *= 1.f / D;
+    mu[tid] = m;
+  }
+  __syncthreads();
+  for (int f = tid; f < D; f += 256) d0s[f] = tile[f] - mu[0];
+  __syncthreads();
+  float dot = 0.f, sq = 0.f;
+  if (tid < Rp) {
+    const float* row = tile + tid * SD;
+#pragma unroll 8
+    for (int f = 0; f < D; ++f) {
+      const float a0 = row[f] - m;
+      dot += a0 * d0s[f];
+      sq += a0 * a0;
+    }
+  }
+  const bool jrow = tid >= ROW_J0 && tid < R;
+  float p_lap = jrow ? sq : 0.f;
+  if (tid == ROW_LAP) p_lap += dot;
+  p_lap = wave_sum(p_lap);
+  if ((tid & 63) == 0) red[tid >> 6] = p_lap;
+  if (tid == 0) red[4] = dot;
+  __syncthreads();
+  const float v = red[4] / D;
+  const float lapv = 2.f / D * (red[0] + red[1] + red[2] + red[3]);
+  const float ve = v + eps;
+  const float sv = rsqrtf(ve);
+  const float s1 = -0.5f * sv / ve;
+  const float s2 = 0.75f * sv / (ve * ve);
+  const float jv = jrow ? 2.f / D * dot : 0.f;
+  float p_jv2 = wave_sum(jv * jv);
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = p_jv2;
+  js[tid] = s1 * jv;
+  __syncthreads();
+  const float laps = s1 * lapv + s2 * (red[0] + red[1] + red[2] + red[3]);
+  float* yu = y + base;
+  for (int f = tid; f < D; f += 256) {
+    const float g = scale[f], d0 = d0s[f];
+    float cross = 0.f;
+#pragma unroll 8
+    for (int r = ROW_J0; r < R; ++r) {
+      const float dr = tile[r * SD + f] - mu[r];
+      cross += dr * js[r];
+      yu[(long)r * D + f] = g * (dr * sv + d0 * js[r]);
+    }
+    for (int r = R > ROW_J0 ? R : ROW_J0; r < Rp; ++r) yu[(long)r * D + f] = 0.f;
+    yu[f] = g * d0 * sv + bias[f];
+    if (Rp > 1) {
+      const float dl = tile[SD + f] - mu[1];
       yu[D + f] = g * (dl * sv + d0 * laps + 2.f * cross);
     }
   }
@@ -266,9 +354,32 @@ int launch_psi_features(const FermiDims& d, const float* elec, const float* atom
   return 0;
 }
 
+template <int D>
+bool launch_ln_lds(int units, int Rp, int R, const float* x, float* y, const float* scale, const float* bias,
+                   float eps, hipStream_t s) {
+  const size_t smem = (size_t)Rp * (D + 1) * sizeof(float);
+  const size_t fixed = (D + 2 * 256 + 8) * sizeof(float);
+  if (smem + fixed > 160 * 1024) return false;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)layernorm_lap_lds_kernel<D>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(160 * 1024 - fixed));
+    attr = true;
+  }
+  hipLaunchKernelGGL(layernorm_lap_lds_kernel<D>, dim3(units), dim3(256), smem, s, Rp, R, x, y, scale, bias, eps);
+  return true;
+}
+
 int launch_layernorm_lap(int units, int Rp, int R, int D, const float* x, float* y, const float* scale,
                          const float* bias, float eps, hipStream_t s) {
   if (Rp > 256) return -1;
+  if (Rp > 1 && units > 0) {  // Laplacian mode: tile resident in LDS when it fits
+    bool ok = false;
+    if (D == 64) ok = launch_ln_lds<64>(units, Rp, R, x, y, scale, bias, eps, s);
+    if (D == 128) ok = launch_ln_lds<128>(units, Rp, R, x, y, scale, bias, eps, s);
+    if (D == 256) ok = launch_ln_lds<256>(units, Rp, R, x, y, scale, bias, eps, s);
+    if (ok) return 0;
+  }
   switch (D) {
     case 64: hipLaunchKernelGGL(layernorm_lap_kernel<64>, dim3(units), dim3(256), 0, s, Rp, R, x, y, scale, bias, eps); return 0;
     case 128: hipLaunchKernelGGL(layernorm_lap_kernel<128>, dim3(units), dim3(256), 0, s, Rp, R, x, y, scale, bias, eps); return 0;
