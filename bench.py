@@ -223,9 +223,22 @@ def main():
     tot_ms, n = mixed["prof"]["gemm_layer_lap"]
     achieved = fl["gemm_layer_lap"] * B * args.steps / (tot_ms * 1e-3) if n else 0.0
     peak = RF.PEAKS["bf16"]
+    # Context for the fraction: each FP32-accurate product costs `mult` BF16
+    # MFMA products, so issued MFMA work is mult x the algorithmic FLOPs; the
+    # minimum HBM bytes (operands once, outputs once) give the memory side.
+    # Algorithmic intensity is ~60 FLOP/B, above the emulated ridge
+    # (2.5 PFLOP/s / mult / 8 TB/s = 52 FLOP/B for bf16x6), so MFMA bounds it.
+    mult = RF.MFMA_PER_PRODUCT[args.mixed]
+    alg_bytes = RF.layer_bytes(spec, system.atoms.shape[0]) * B * args.steps
+    traffic, traffic_src = measured_traffic(args.workload)
+    avg_s = tot_ms * 1e-3 / max(n, 1)
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": measured_traffic(args.workload)[0],
-            "traffic_unit": "bytes/launch", "traffic_source": measured_traffic(args.workload)[1],
+            "frac": achieved / peak, "traffic": traffic,
+            "traffic_unit": "bytes/launch", "traffic_source": traffic_src,
+            "mfma_per_product": mult, "issued_mfma_frac": achieved * mult / peak,
+            "alg_bytes_per_launch": alg_bytes / max(n, 1),
+            "hbm_frac_alg_bytes": (alg_bytes / (tot_ms * 1e-3) / RF.PEAKS["hbm"]) if n else 0.0,
+            "hbm_frac_measured_traffic": (traffic / avg_s / RF.PEAKS["hbm"]) if (n and traffic) else None,
             "kernel": f"gemm_kernel ({spec.kind} dense layers, Laplacian mode, {args.mixed})",
             "avg_launch_ms": tot_ms / max(n, 1), "launches": n,
             "per_class_ms_per_step": {k: v[0] / args.steps for k, v in mixed["prof"].items()},

@@ -6,7 +6,7 @@
 // spin-mean broadcast term, the orbital projection and the Psiformer
 // projections.  Both operands are K-contiguous FP32 in HBM and staged through
 // LDS in K-chunks of 32, with the next chunk prefetched into registers while
-// the current one is multiplied.  Each of the 4 waves owns NBW 16-column blocks
+// the current one is multiplied (weights: see the K loop).  Each of the 4 waves owns NBW 16-column blocks
 // of the tile and *all* of its rows, so the cross-row reduction sum_c J_c^2 of
 // the tanh rule stays inside one wavefront (two xor-shuffles).
 //
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // the epilogue reuses it to stage output rows.  The weights are not staged:
   // every wave owns its own columns, so each lane loads its MFMA B fragments
   // (8 consecutive k of one column) straight from global memory, where the
-  // weights stay L2-resident, one K chunk ahead.
+  // weights stay L2-resident.
   constexpr int A_BYTES = NSL * BM * LS * (int)sizeof(TL);
   constexpr int STG_BYTES = 4 * 16 * (NBW * 16 + 4) * 4;
   constexpr int ARENA = A_BYTES > STG_BYTES ? A_BYTES : STG_BYTES;
@@ -220,7 +220,11 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   bload(0);
   for (int k0 = 0; k0 < a.K; k0 += KC) {
     lstore();
-    // this chunk's B fragments; the next chunk's are loaded meanwhile
+    // this chunk's B fragments.  FP32: the next chunk's are loaded while this
+    // one is multiplied.  BF16 modes: the NS planes are used from the load
+    // registers and the next chunk's are requested after the last MFMA (they
+    // are L2 hits and arrive during the next staging step), which keeps the
+    // 128-row tiles at three waves per SIMD.
     float bf[NBW][8];
     bf16x8 bt[NS][NBW];
     if constexpr (F32) {
@@ -229,16 +233,17 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         *(float4*)&bf[nb][0] = bn32[nb][0];
         *(float4*)&bf[nb][4] = bn32[nb][1];
       }
-    } else {
-#pragma unroll
-      for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int nb = 0; nb < NBW; ++nb) bt[s][nb] = __builtin_bit_cast(bf16x8, bnb[s][nb]);
     }
     __syncthreads();
     if (k0 + KC < a.K) {
       gload(k0 + KC);
-      bload(k0 + KC);
+      if constexpr (F32) bload(k0 + KC);
+    }
+    if constexpr (!F32) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) bt[s][nb] = __builtin_bit_cast(bf16x8, bnb[s][nb]);
     }
 
     if constexpr (F32) {
@@ -275,6 +280,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
           acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0][nb], acc[mb][nb], 0, 0, 0);
         }
       }
+      if (k0 + KC < a.K) bload(k0 + KC);
     }
     __syncthreads();
   }

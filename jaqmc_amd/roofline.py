@@ -87,3 +87,49 @@ def psiformer_flops(spec: ModelSpec, n_atoms: int):
 
 def flops(spec: ModelSpec, n_atoms: int):
     return ferminet_flops(spec, n_atoms) if spec.kind == "ferminet" else psiformer_flops(spec, n_atoms)
+
+
+def ferminet_layer_bytes(spec: ModelSpec, n_atoms: int):
+    """Per-walker HBM bytes the Laplacian-mode layer GEMMs must move at least:
+    each operand read once and each output written once, FP32, on the C
+    algorithmic tangent rows (the residual is a slice of the input, so it adds
+    nothing; weights are per launch, not per walker)."""
+    N = spec.n_elec
+    C = 3 * N + 2
+    nch = spec.n_channels
+    f1, f2 = 4 * n_atoms, 4
+    tot = 0.0
+    L = len(spec.hidden_single)
+    for l, (d1, d2) in enumerate(zip(spec.hidden_single, spec.hidden_double)):
+        rows1 = 5 if l == 0 else C
+        tot += 4.0 * N * (rows1 * f1 + C * nch * f2)  # one-electron input and pair means
+        tot += 4.0 * N * C * d1 + 4.0 * C * d1  # output rows and the spin-mean broadcast term
+        if l < L - 1:
+            f2 = d2
+        f1 = d1
+    return tot
+
+
+def psiformer_layer_bytes(spec: ModelSpec, n_atoms: int):
+    """Per-walker minimum HBM bytes of the Psiformer Laplacian-mode projections:
+    Q/K/V (input, 3D output), output projection and last MLP layer (input,
+    residual stream read, residual stream written), hidden MLP layers."""
+    N = spec.n_elec
+    C = 3 * N + 2
+    D = spec.num_heads * spec.heads_dim
+    tot = 4.0 * N * (5 * (4 * n_atoms + 1) + C * D)
+    for _ in range(spec.num_layers):
+        tot += 4.0 * N * C * (D + 3 * D)
+        tot += 4.0 * N * C * 3 * D
+        dims = [D] + list(spec.mlp_hidden) + [D]
+        for j, (a, b) in enumerate(zip(dims[:-1], dims[1:])):
+            tot += 4.0 * N * C * (a + b + (D if j == len(dims) - 2 else 0))
+    return tot
+
+
+def layer_bytes(spec: ModelSpec, n_atoms: int):
+    return (ferminet_layer_bytes(spec, n_atoms) if spec.kind == "ferminet"
+            else psiformer_layer_bytes(spec, n_atoms))
+
+
+MFMA_PER_PRODUCT = {"fp32": 1, "bf16": 1, "bf16x3": 3, "bf16x6": 6}
