@@ -43,8 +43,19 @@ struct AttnLayout {
   static constexpr int NP = 16 * NT;   // padded electrons
   static constexpr int SQ = DH + 4;    // Q, K rows [i][d]
   static constexpr int SJ = NP + 4;    // [i][j] arrays and transposed V [d][j]
-  static constexpr int FLOATS = 4 * NP * SQ + 2 * DH * SJ + 3 * NP * SJ;
+  static constexpr int FLOATS = 4 * NP * SQ + 2 * DH * SJ + 3 * NP * SJ + NT * NP;
 };
+
+// Sum over the 16 lanes of each DPP row (lanes 16r .. 16r+15), result in every
+// lane of the row: quad_perm [1,0,3,2] and [2,3,0,1] (xor 1, xor 2), then
+// row_half_mirror and row_mirror fold the quads and the halves.
+__device__ inline float row16_sum(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x141, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x140, 0xF, 0xF, false));
+  return v;
+}
 
 template <int NT, int DH>
 __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
@@ -66,8 +77,10 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
   float* V0 = Kc + NP * SQ;   // V_0 transposed [d][j]
   float* Vc = V0 + DH * SJ;   // V_c transposed
   float* As = Vc + DH * SJ;   // attention weights a[i][j]
-  float* Jt = As + NP * SJ;   // J_c, then Ja_c (then S3, then lap a)
+  float* Jt = As + NP * SJ;   // Ja_c (then S3, then lap a)
   float* J1 = Jt + NP * SJ;   // J_1
+  float* Mp = J1 + NP * SJ;   // per column tile partial row sums sum_j a J_c  [Qt][i]
+  float* RJ = Q0;             // after the tangent loop: rj [i][j]
 
   const int N = a.N, RP = a.Rp, R = a.R;
   const int z = blockIdx.x, w = z / a.H, h = z % a.H;
@@ -79,13 +92,30 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
 
   for (int e = tid; e < L::FLOATS; e += ATT_THREADS) sm[e] = 0.f;  // padding rows and columns stay zero
 
-  // row c of Q, K, V (rows i < N) through registers
+  // row c of Q, K, V (rows i < N) through registers.  Item `it` is float4 q
+  // of row i of part m.  Q and K items run along the row (16-byte accesses to
+  // consecutive addresses); V items run four along the row, then down the
+  // rows, so that the transposed LDS writes of a wave hit 16 consecutive
+  // columns j per quarter-row instead of two banks.
+  static_assert(F4 % 4 == 0, "head dimension must be a multiple of 16");
+  auto item = [&](int it, int& m, int& i, int& q) __attribute__((always_inline)) {
+    m = it / (NP * F4);
+    const int rem = it % (NP * F4);
+    if (m < 2) {
+      i = rem / F4;
+      q = rem % F4;
+    } else {
+      const int rest = rem >> 2;
+      i = rest % NP;
+      q = (rest / NP) * 4 + (rem & 3);
+    }
+  };
   float4 pf[PER];
   auto fetch = [&](int c) __attribute__((always_inline)) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      const int it = tid + ATT_THREADS * p;
-      const int m = it / (NP * F4), rem = it % (NP * F4), i = rem / F4, q = rem % F4;
+      int m, i, q;
+      item(tid + ATT_THREADS * p, m, i, q);
       pf[p] = (m < 3 && i < N && c < R) ? *(const float4*)(qsrc + m * pstride + ((long)i * RP + c) * DH + 4 * q)
                                         : float4{0.f, 0.f, 0.f, 0.f};
     }
@@ -94,8 +124,8 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
   auto commit = [&](float* dq, float* dv) __attribute__((always_inline)) {
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-      const int it = tid + ATT_THREADS * p;
-      const int m = it / (NP * F4), rem = it % (NP * F4), i = rem / F4, q = rem % F4;
+      int m, i, q;
+      item(tid + ATT_THREADS * p, m, i, q);
       if (m < 2 && i < N) {
         *(float4*)(dq + (m * NP + i) * SQ + 4 * q) = pf[p];
       } else if (m == 2 && i < N) {
@@ -181,14 +211,28 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
   }
 
   // ---- tangent rows ----
-  f32x4 s3[JTW], l6[OTW];
+  // The score tiles stay in the MFMA layout through the centring: a wave
+  // keeps its J tiles' a values, the rj = sum_c delta_c^2 accumulators and the
+  // tiles' validity in registers; row sums sum_j a J_c are 16-lane DPP sums
+  // per tile, combined over the NT column tiles through LDS.
+  f32x4 s3[JTW], l6[OTW], rjt[JTW], at[JTW];
 #pragma unroll
-  for (int t = 0; t < JTW; ++t) s3[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < JTW; ++t) {
+    s3[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    rjt[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
 #pragma unroll
   for (int t = 0; t < OTW; ++t) l6[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float rj[RPW];
+  __syncthreads();  // attention weights final
 #pragma unroll
-  for (int t = 0; t < RPW; ++t) rj[t] = 0.f;
+  for (int t = 0; t < JTW; ++t) {
+    const int tile = wave + ATT_WAVES * t;
+    if (tile < JT) {
+      const int P = tile / NT, Qt = tile % NT;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) at[t][v] = As[(16 * P + 4 * lg + v) * SJ + 16 * Qt + lr];
+    }
+  }
 
   for (int c = 1; c < R; ++c) {
     __syncthreads();  // previous row's readers of Qc, Kc, Vc and Jt are done
@@ -215,26 +259,46 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
             if (cross) mfma4(s3[t], qc, kc);
           }
         }
-      float* Jdst = c == 1 ? J1 : Jt;
+      if (c == 1) {
+#pragma unroll
+        for (int t = 0; t < JTW; ++t) {
+          const int tile = wave + ATT_WAVES * t;
+          if (tile < JT) put_tile(J1, acc[t], tile / NT, tile % NT);
+        }
+        continue;  // V_1 enters the Laplacian row after the loop
+      }
+      // centre and weight: Ja_c = a (J_c - mu_c), rj += (J_c - mu_c)^2
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
         const int tile = wave + ATT_WAVES * t;
-        if (tile < JT) put_tile(Jdst, acc[t], tile / NT, tile % NT);
-      }
-    }
-    if (c == 1) continue;  // V_1 enters the Laplacian row after the loop
-    __syncthreads();
-    // centre and weight: Jt <- a (J_c - mu_c), rj += (J_c - mu_c)^2
+        if (tile < JT) {
+          const int P = tile / NT, Qt = tile % NT;
 #pragma unroll
-    for (int t = 0; t < RPW; ++t) {
-      const int i = wave + ATT_WAVES * t;
-      if (i < N) {
-        const float jv = jok ? Jt[i * SJ + lane] : 0.f;
-        const float av = jok ? As[i * SJ + lane] : 0.f;
-        const float mu = wave_sum(av * jv);
-        const float d = jok ? jv - mu : 0.f;
-        if (jok) Jt[i * SJ + lane] = av * d;
-        rj[t] += d * d;
+          for (int v = 0; v < 4; ++v) {
+            const float rs = row16_sum(at[t][v] * acc[t][v]);
+            if (lr == 0) Mp[Qt * NP + 16 * P + 4 * lg + v] = rs;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < JTW; ++t) {
+        const int tile = wave + ATT_WAVES * t;
+        if (tile < JT) {
+          const int P = tile / NT, Qt = tile % NT;
+          const bool jin = 16 * Qt + lr < N;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * P + 4 * lg + v;
+            float mu = 0.f;
+#pragma unroll
+            for (int q = 0; q < NT; ++q) mu += Mp[q * NP + i];
+            const float d = (jin && i < N) ? acc[t][v] - mu : 0.f;
+            rjt[t][v] += d * d;
+            acc[t][v] = at[t][v] * d;
+          }
+          put_tile(Jt, acc[t], P, Qt);
+        }
       }
     }
     __syncthreads();
@@ -271,7 +335,10 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
 #pragma unroll
   for (int t = 0; t < JTW; ++t) {
     const int tile = wave + ATT_WAVES * t;
-    if (tile < JT) put_tile(Jt, s3[t], tile / NT, tile % NT);
+    if (tile < JT) {
+      put_tile(Jt, s3[t], tile / NT, tile % NT);
+      put_tile(RJ, rjt[t], tile / NT, tile % NT);
+    }
   }
   commit(Qc, Vc);
   __syncthreads();
@@ -282,7 +349,7 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
     if (i < N) {
       const float av = jok ? As[i * SJ + lane] : 0.f;
       const float ls = jok ? J1[i * SJ + lane] + 2.f * Jt[i * SJ + lane] : 0.f;
-      const float r = jok ? rj[t] : 0.f;
+      const float r = jok ? RJ[i * SJ + lane] : 0.f;
       const float nu = wave_sum(av * r), lam = wave_sum(av * ls);
       if (jok) Jt[i * SJ + lane] = av * (r - nu + ls - lam);
     }

@@ -510,15 +510,16 @@ bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
 }
 
 // Default (UPT, NBW) per unit height and precision, from tile sweeps on
-// MI355X (K = 256-320, 256 output columns).
+// MI355X (K = 256-320, 256 output columns).  BF16-split modes at 48-row units
+// take four units per tile (the weight fragments of a chunk are reused over
+// 12 row blocks), FP32 two (three waves per SIMD).
 void default_cfg(int R, int prec, bool wide, int* upt, int* nbw) {
-  (void)prec;
   (void)wide;
   *nbw = 2;
   switch (R) {
     case 16: *upt = 4; break;
     case 32: *upt = 2; break;
-    case 48: *upt = 2; break;
+    case 48: *upt = prec == PREC_F32 ? 2 : 4; break;
     case 64: *upt = 2; break;
     default: *upt = 1; break;
   }
