@@ -16,13 +16,13 @@
 //   O_0 = a V_0,  O_c = Ja_c V_0 + a V_c  (c >= 2),
 //   O_1 = lap_a V_0 + a V_1 + 2 sum_{c>=2} Ja_c V_c.
 // The workgroup stages the value rows Q_0, K_0, V_0 and the attention
-// weights once, then streams the tangent rows: for each c it stages
-// Q_c, K_c, V_c (the next c is prefetched into registers meanwhile), forms
-// the 16x16 tiles of J_c (and accumulates S3) with v_mfma_f32_16x16x4f32,
-// centres and weights them row by row (one wave per query row, lanes over
-// keys, shuffles for the sums), forms O_c (and accumulates sum_c Ja_c V_c)
-// and writes O_c straight to the attention output.  The score tensors never
-// leave LDS.
+// weights once, then streams the tangent rows through two LDS row buffers:
+// row c is multiplied while row c + 1 is staged and row c + 2 is in flight
+// from HBM.  For each c it forms the 16x16 tiles of J_c (and accumulates S3)
+// with v_mfma_f32_16x16x4f32, centres and weights them in the MFMA layout
+// (16-lane DPP row sums), forms O_c (and accumulates sum_c Ja_c V_c) and
+// writes O_c straight to the attention output; two workgroup barriers per
+// row.  The score tensors never leave LDS.
 //
 // Eight waves share each product's 16x16 output tiles.  MFMA 16x16x4 (f32):
 // lane (r = lane%16, g = lane/16) supplies A[r][k] and B[k][r]; result
@@ -43,7 +43,9 @@ struct AttnLayout {
   static constexpr int NP = 16 * NT;   // padded electrons
   static constexpr int SQ = DH + 4;    // Q, K rows [i][d]
   static constexpr int SJ = NP + 4;    // [i][j] arrays and transposed V [d][j]
-  static constexpr int FLOATS = 4 * NP * SQ + 2 * DH * SJ + 3 * NP * SJ + NT * NP;
+  // Q_0, K_0, two row buffers of (Q_c, K_c); V_0 and two row buffers of V_c;
+  // a, Ja_c, J_1; the partial row sums
+  static constexpr int FLOATS = 6 * NP * SQ + 3 * DH * SJ + 3 * NP * SJ + NT * NP;
 };
 
 // Sum over the 16 lanes of each DPP row (lanes 16r .. 16r+15), result in every
@@ -72,11 +74,10 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
   extern __shared__ float sm[];
   float* Q0 = sm;             // Q_0, then K_0 at + NP*SQ
   float* K0 = Q0 + NP * SQ;
-  float* Qc = K0 + NP * SQ;   // Q_c, then K_c
-  float* Kc = Qc + NP * SQ;
-  float* V0 = Kc + NP * SQ;   // V_0 transposed [d][j]
-  float* Vc = V0 + DH * SJ;   // V_c transposed
-  float* As = Vc + DH * SJ;   // attention weights a[i][j]
+  float* Qb = K0 + NP * SQ;   // row c in buffer c % 2: Q_c at Qb + 2*NP*SQ*(c%2), K_c after it
+  float* V0 = Qb + 4 * NP * SQ;  // V_0 transposed [d][j]
+  float* Vb = V0 + DH * SJ;   // V_c transposed, buffer c % 2 at Vb + DH*SJ*(c%2)
+  float* As = Vb + 2 * DH * SJ;  // attention weights a[i][j]
   float* Jt = As + NP * SJ;   // Ja_c (then S3, then lap a)
   float* J1 = Jt + NP * SJ;   // J_1
   float* Mp = J1 + NP * SJ;   // per column tile partial row sums sum_j a J_c  [Qt][i]
@@ -196,6 +197,8 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
       if (tile < JT) put_tile(As, acc[t], tile / NT, tile % NT);
     }
   }
+  commit(Qb + 2 * NP * SQ, Vb + DH * SJ);  // row 1 into buffer 1
+  fetch(2);
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < RPW; ++t) {
@@ -234,40 +237,36 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
     }
   }
 
+  // Row c lives in buffer c % 2.  Per row: J products -> partial row sums ->
+  // barrier -> centring, commit of row c + 1 into the other buffer, fetch of
+  // row c + 2 -> barrier -> O products.  The other buffer's last readers (the
+  // products of row c - 1) are behind the first barrier, and row c + 1 is in
+  // LDS before the second.
   for (int c = 1; c < R; ++c) {
-    __syncthreads();  // previous row's readers of Qc, Kc, Vc and Jt are done
-    commit(Qc, Vc);
-    fetch(c + 1);
-    __syncthreads();
-    {
-      // J_c = Q_c K_0^T + Q_0 K_c^T  and  S3 += Q_c K_c^T
-      f32x4 acc[JTW];
+    const int b = c & 1;
+    const float* Qc = Qb + 2 * NP * SQ * b;
+    const float* Kc = Qc + NP * SQ;
+    const float* Vc = Vb + DH * SJ * b;
+    // J_c = Q_c K_0^T + Q_0 K_c^T  and  S3 += Q_c K_c^T
+    f32x4 acc[JTW];
 #pragma unroll
-      for (int t = 0; t < JTW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-      const bool cross = c >= ROW_J0;
+    for (int t = 0; t < JTW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bool cross = c >= ROW_J0;
 #pragma unroll
-      for (int g = 0; g < KQ / 4; ++g)
+    for (int g = 0; g < KQ / 4; ++g)
 #pragma unroll
-        for (int t = 0; t < JTW; ++t) {
-          const int tile = wave + ATT_WAVES * t;
-          if (tile < JT) {
-            const int P = tile / NT, Qt = tile % NT;
-            const float4 qc = rd4(Qc, SQ, 16 * P + lr, lg * KQ, g), q0 = rd4(Q0, SQ, 16 * P + lr, lg * KQ, g);
-            const float4 k0 = rd4(K0, SQ, 16 * Qt + lr, lg * KQ, g), kc = rd4(Kc, SQ, 16 * Qt + lr, lg * KQ, g);
-            mfma4(acc[t], qc, k0);
-            mfma4(acc[t], q0, kc);
-            if (cross) mfma4(s3[t], qc, kc);
-          }
+      for (int t = 0; t < JTW; ++t) {
+        const int tile = wave + ATT_WAVES * t;
+        if (tile < JT) {
+          const int P = tile / NT, Qt = tile % NT;
+          const float4 qc = rd4(Qc, SQ, 16 * P + lr, lg * KQ, g), q0 = rd4(Q0, SQ, 16 * P + lr, lg * KQ, g);
+          const float4 k0 = rd4(K0, SQ, 16 * Qt + lr, lg * KQ, g), kc = rd4(Kc, SQ, 16 * Qt + lr, lg * KQ, g);
+          mfma4(acc[t], qc, k0);
+          mfma4(acc[t], q0, kc);
+          if (cross) mfma4(s3[t], qc, kc);
         }
-      if (c == 1) {
-#pragma unroll
-        for (int t = 0; t < JTW; ++t) {
-          const int tile = wave + ATT_WAVES * t;
-          if (tile < JT) put_tile(J1, acc[t], tile / NT, tile % NT);
-        }
-        continue;  // V_1 enters the Laplacian row after the loop
       }
-      // centre and weight: Ja_c = a (J_c - mu_c), rj += (J_c - mu_c)^2
+    if (c >= ROW_J0) {
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
         const int tile = wave + ATT_WAVES * t;
@@ -280,7 +279,17 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
           }
         }
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    if (c == 1) {
+      // J_1 is kept for the Laplacian row; V_1 enters it after the loop
+#pragma unroll
+      for (int t = 0; t < JTW; ++t) {
+        const int tile = wave + ATT_WAVES * t;
+        if (tile < JT) put_tile(J1, acc[t], tile / NT, tile % NT);
+      }
+    } else {
+      // centre and weight: Ja_c = a (J_c - mu_c), rj += (J_c - mu_c)^2
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
         const int tile = wave + ATT_WAVES * t;
@@ -301,7 +310,12 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
         }
       }
     }
+    if (c + 1 < R) {
+      commit(Qb + 2 * NP * SQ * (b ^ 1), Vb + DH * SJ * (b ^ 1));
+      fetch(c + 2);
+    }
     __syncthreads();
+    if (c == 1) continue;
     {
       // O_c = Ja_c V_0 + a V_c,  L6 += Ja_c V_c
       f32x4 acc[OTW];
@@ -340,7 +354,7 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
       put_tile(RJ, rjt[t], tile / NT, tile % NT);
     }
   }
-  commit(Qc, Vc);
+  commit(Qb, Vb);  // row 1 into buffer 0 (its readers are done)
   __syncthreads();
   // lap s = J_1 + 2 S3; lap a = a (rj - nu + lap s - lam)
 #pragma unroll
@@ -364,7 +378,7 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
     }
     av_tiles(o0, As, V0);
     av_tiles(o1, Jt, V0);
-    av_tiles(o1, As, Vc);
+    av_tiles(o1, As, Vb);
 #pragma unroll
     for (int t = 0; t < OTW; ++t) {
       const int tile = wave + ATT_WAVES * t;
