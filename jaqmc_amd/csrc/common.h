@@ -46,6 +46,20 @@ template <typename T> __device__ inline T from_f(float x);
 template <> __device__ inline float from_f<float>(float x) { return x; }
 template <> __device__ inline bf16 from_f<bf16>(float x) { return (bf16)x; }
 
+// Workgroup -> (row tile, column block).  Workgroups are dispatched to the
+// eight XCDs round-robin (workgroup b runs on XCD b % 8, each XCD with its own
+// L2), so the column blocks of one row tile go to b, b + 8, b + 16, ...: they
+// run back to back on one XCD and the row tile's shared operand is read from
+// HBM once instead of once per column block.  Row tiles past the end return
+// at once.
+constexpr int kXcds = 8;
+__host__ __device__ inline void xcd_tile(int b, int ncb, int* rt, int* cb) {
+  const int x = b % kXcds, slot = b / kXcds;
+  *cb = slot % ncb;
+  *rt = (slot / ncb) * kXcds + x;
+}
+inline int xcd_grid(long nrt, int ncb) { return (int)((nrt + kXcds - 1) / kXcds) * kXcds * ncb; }
+
 __device__ inline float wave_sum(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);

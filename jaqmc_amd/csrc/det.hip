@@ -297,7 +297,11 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
   float* E0 = (float*)(red + 8);  // NP*ES envelope values env[i][o]
   int* ipiv = (int*)(E0 + NP * ES);
   const int N = d.N, T = 3 * N;
-  const int w = blockIdx.x / d.K, k = blockIdx.x % d.K;
+  // the K determinants of a walker share rows of Phi: same XCD (L2)
+  int w, k;
+  xcd_tile(blockIdx.x, d.K, &w, &k);
+  if (w >= d.Bc) return;
+  const long det_id = (long)w * d.K + k;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long rowstride = (long)d.Rp * ldphi;  // floats per unit
   const long kN = (long)k * N;
@@ -376,7 +380,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
     }
     __syncthreads();
   }
-  double* o_ = out + (long)blockIdx.x * (3 + T);
+  double* o_ = out + det_id * (3 + T);
 
   // tr(A^-1 L), L = lap(phi env) = lap(phi) env + phi lap(env) + 2 grad(phi).grad(env)
   double part = 0.0;
@@ -501,11 +505,15 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
   constexpr int NB8 = NP / 8;
   extern __shared__ double smf[];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const long det = (long)blockIdx.x * 2 + wv;
-  if (det >= ndet) return;  // the wave's own determinant; nothing is shared across waves
+  // determinants 2cb, 2cb + 1 of walker w; a walker's blocks share an XCD
+  int w, cb;
+  xcd_tile(blockIdx.x, (d.K + 1) / 2, &w, &cb);
+  const int k = 2 * cb + wv;
+  if (w >= d.Bc || k >= d.K) return;  // the wave's own determinant; nothing is shared across waves
+  const long det = (long)w * d.K + k;
+  (void)ndet;
   double* M = smf + wv * NP * S;
   const int N = d.N;
-  const int w = (int)(det / d.K), k = (int)(det % d.K);
 #pragma unroll 8
   for (int e = lane; e < N * N; e += 64) {
     const int i = e / N, o = e - i * N;
@@ -566,19 +574,42 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
 // ---------------------------------------------------------------------------
 // Small systems (N <= 16): one wavefront per determinant, four per workgroup,
 // no workgroup barriers.  The augmented [A | I] (16 x 32 FP64) is reduced by
-// Gauss-Jordan with partial pivoting in the wave's LDS slice; then lane
-// (a = lane/4, jb = lane%4) keeps inverse row a in registers and forms
-// B_c[a][4jb..4jb+3] = sum_i Inv[a][i] J_c[i][.] for each coordinate c, with
+// Gauss-Jordan with partial pivoting in the wave's LDS slice; then, for each
+// coordinate c, B_c = A^-1 J_c is one 16x16 FP64 MFMA tile (four K steps,
+// A^-1 fragments in registers, J_c from HBM several coordinates ahead), with
 // tr(B_c) from the diagonal lanes and tr(B_c^2) from a transposed read of the
-// wave's B_c tile.  The next J_c is prefetched from HBM while B_c is formed.
+// tile in the wave's LDS slice.
 // ---------------------------------------------------------------------------
 
+// 64-bit DPP move (both halves with the same control)
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline double readlane_d(double x, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// Wavefront sum of a double, uniform result: DPP sums within each 16-lane row
+// (xor 1, xor 2, half-row mirror, row mirror), then the four row totals.
+__device__ inline double wave_sum_dpp_d(double x) {
+  x += dpp_d<0xB1>(x);
+  x += dpp_d<0x4E>(x);
+  x += dpp_d<0x141>(x);
+  x += dpp_d<0x140>(x);
+  return readlane_d(x, 0) + readlane_d(x, 16) + readlane_d(x, 32) + readlane_d(x, 48);
+}
+
 constexpr int SM_NP = 16;
+constexpr int SM_PF = 4;  // coordinates in flight in the small kernel (even)
 struct SmallDetSmem {
   double M[SM_NP * 2 * SM_NP];  // augmented matrix, later Inv in columns 16..31
-  double J[SM_NP * SM_NP];
-  double Bt[SM_NP * SM_NP];
-  double grad[3 * SM_NP];
+  double Tt[2 * SM_NP * 17];    // transpose tiles of B_c, B_c+1 (odd row stride)
 };
 
 __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
@@ -587,9 +618,12 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
   __shared__ SmallDetSmem smem[4];
   const int N = d.N, T = 3 * N;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long det_id = (long)blockIdx.x * 4 + wave;
-  if (det_id >= (long)d.Bc * d.K) return;  // whole wave exits together
-  const int w = (int)(det_id / d.K), k = (int)(det_id % d.K);
+  // determinants 4cb .. 4cb + 3 of walker w; a walker's blocks share an XCD
+  int w, cb;
+  xcd_tile(blockIdx.x, (d.K + 3) / 4, &w, &cb);
+  const int k = 4 * cb + wave;
+  if (w >= d.Bc || k >= d.K) return;  // whole wave exits together
+  const long det_id = (long)w * d.K + k;
   SmallDetSmem& S = smem[wave];
   const int Rp = d.Rp;
 
@@ -625,18 +659,22 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
 
   double logdet = 0.0;
   int sgn = 1;
+  // pivot search over rows p..15 in lanes 0..15: DPP butterfly within the
+  // first 16-lane row (largest |value|, smallest row on ties)
+  auto pick = [&](double& v, int& bi, double ov, int oi) __attribute__((always_inline)) {
+    if (ov > v || (ov == v && oi < bi)) {
+      v = ov;
+      bi = oi;
+    }
+  };
   for (int p = 0; p < SM_NP; ++p) {
     double v = (lane >= p && lane < SM_NP) ? fabs(S.M[lane * 32 + p]) : -1.0;
     int bi = lane;
-    for (int o = 1; o < 16; o <<= 1) {
-      const double ov = __shfl_xor(v, o);
-      const int oi = __shfl_xor(bi, o);
-      if (ov > v || (ov == v && oi < bi)) {
-        v = ov;
-        bi = oi;
-      }
-    }
-    const int q = __shfl(bi, 0);
+    pick(v, bi, dpp_d<0xB1>(v), __builtin_amdgcn_mov_dpp(bi, 0xB1, 0xF, 0xF, false));
+    pick(v, bi, dpp_d<0x4E>(v), __builtin_amdgcn_mov_dpp(bi, 0x4E, 0xF, 0xF, false));
+    pick(v, bi, dpp_d<0x141>(v), __builtin_amdgcn_mov_dpp(bi, 0x141, 0xF, 0xF, false));
+    pick(v, bi, dpp_d<0x140>(v), __builtin_amdgcn_mov_dpp(bi, 0x140, 0xF, 0xF, false));
+    const int q = __builtin_amdgcn_readlane(bi, 0);
     if (q != p) {
       if (lane < 32) {
         const double t0 = S.M[p * 32 + lane];
@@ -675,13 +713,6 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
     return;
   }
 
-  // inverse row a in registers
-  const int a = lane >> 2, jb = (lane & 3) * 4;
-  double inv[SM_NP];
-#pragma unroll
-  for (int i = 0; i < SM_NP; ++i) inv[i] = S.M[a * 32 + 16 + i];
-  for (int c = lane; c < T; c += 64) S.grad[c] = 0.0;
-
   // tr(A^-1 L): lane's 4 elements (i, o) pair with Inv[o][i]
   double part = 0.0;
 #pragma unroll
@@ -694,49 +725,98 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
     part += S.M[o * 32 + 16 + i] * L;
   }
   const double trAL = wave_sum_d(part);
-  wave_sync();
 
-  float nxt[4];
+  // Coordinate loop on v_mfma_f64_16x16x4f64: B_c = A^-1 J_c in four K
+  // steps.  Lane (r, g) = (lane % 16, lane / 16) supplies A^-1[r][4kk + g]
+  // (constant, in registers) and J_c[4kk + g][r] (from HBM, one coordinate
+  // ahead); result element v is B_c[g + 4v][r].  The envelope-derivative part
+  // of J_c lives in row m* = c / 3 only and enters as B_c += A^-1[:, m*] u^T.
+  const int lr = lane & 15, lg = lane >> 4;
+  // Coordinates c .. c + SM_PF - 1 are in flight from HBM: slot q of the
+  // unrolled loop holds coordinate c0 + q and is refilled with c0 + q + SM_PF
+  // once consumed (one coordinate of MFMA work is far shorter than an HBM
+  // round trip).
+  double av[4];
+  long bb[4];
+  float e0b[4], jv[SM_PF][4], uph[SM_PF], ude[SM_PF];
+  // u for coordinate c: phi[m*][r] * d env[m*][r] / dx_c (lanes r < N)
+  auto load_c = [&](int q, int c) __attribute__((always_inline)) {
 #pragma unroll
-  for (int t = 0; t < 4; ++t) nxt[t] = base[t] >= 0 ? Phi[base[t] + ROW_J0 * ldphi] : 0.f;
+    for (int kk = 0; kk < 4; ++kk) jv[q][kk] = (c < T && bb[kk] >= 0) ? Phi[bb[kk] + (ROW_J0 + c) * ldphi] : 0.f;
+    const int ms = c / 3;
+    if (c < T && lr < N) {
+      uph[q] = Phi[(long)unit_idx(d, w, ms) * Rp * ldphi + (long)k * N + lr];
+      ude[q] = env[((((long)w * N + ms) * d.K + k) * N + lr) * 5 + 2 + c % 3];
+    } else {
+      uph[q] = 0.f;
+      ude[q] = 0.f;
+    }
+  };
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    const int m = 4 * kk + lg;
+    av[kk] = S.M[lr * 32 + 16 + m];
+    const bool in = m < N && lr < N;
+    bb[kk] = in ? (long)unit_idx(d, w, m) * Rp * ldphi + (long)k * N + lr : -1;
+    e0b[kk] = in ? env[((((long)w * N + m) * d.K + k) * N + lr) * 5] : 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < SM_PF; ++q) load_c(q, q);
+  double* tile = S.Tt;
   double tsum = 0.0;
-  for (int c = 0; c < T; ++c) {
-    // J_c into LDS (product rule with the envelope: only row i = c/3 gets phi * dEnv)
+  wave_sync();
+  // two coordinates per step: independent MFMA chains, reductions and
+  // transposes interleave, and the step needs two wave barriers
+  double* tileb = tile + 16 * 17;
+  for (int c0 = 0; c0 < T; c0 += SM_PF)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int e = lane + 64 * t, i = e >> 4;
-      double j = (double)nxt[t] * e0[t];
-      if (base[t] >= 0 && c / 3 == i) j += (double)phi0[t] * ej[t][c % 3];
-      S.J[e] = j;
+  for (int q = 0; q < SM_PF; q += 2) {
+    const int ca = c0 + q, cb = ca + 1;
+    if (ca >= T) break;
+    const bool hb = cb < T;
+    double bva[4], bvb[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      bva[kk] = (double)jv[q][kk] * (double)e0b[kk];
+      bvb[kk] = (double)jv[q + 1][kk] * (double)e0b[kk];  // zero when cb >= T
+    }
+    const double ua = (double)uph[q] * (double)ude[q], ub = (double)uph[q + 1] * (double)ude[q + 1];
+    const int msa = ca / 3, msb = hb ? cb / 3 : 0;
+    // refill both slots; coordinates past T load as zeros, so the second
+    // half of a step past the end contributes nothing
+    load_c(q, ca + SM_PF);
+    load_c(q + 1, cb + SM_PF);
+    f64x4 A = f64x4{0.0, 0.0, 0.0, 0.0}, B = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      A = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bva[kk], A, 0, 0, 0);
+      B = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], bvb[kk], B, 0, 0, 0);
+    }
+    double tra = 0.0, trb = 0.0;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      A[v] += S.M[(lg + 4 * v) * 32 + 16 + msa] * ua;
+      B[v] += S.M[(lg + 4 * v) * 32 + 16 + msb] * ub;
+      if (lr == lg + 4 * v) {
+        tra += A[v];
+        trb += B[v];
+      }
+    }
+    tra = wave_sum_dpp_d(tra);
+    trb = wave_sum_dpp_d(trb);
+    if (lane == 0) {
+      o_[3 + ca] = tra;
+      if (hb) o_[3 + cb] = trb;
+    }
+    // tr(B_c^2) = sum_ab B[a][b] B[b][a] through a transposed read of the tile
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      tile[(lg + 4 * v) * 17 + lr] = A[v];
+      tileb[(lg + 4 * v) * 17 + lr] = B[v];
     }
     wave_sync();
-    if (c + 1 < T) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) nxt[t] = base[t] >= 0 ? Phi[base[t] + (ROW_J0 + c + 1) * ldphi] : 0.f;
-    }
-    double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
-#pragma unroll
-    for (int i = 0; i < SM_NP; ++i) {
-      const double* jr = &S.J[i * 16 + jb];
-      b0 += inv[i] * jr[0];
-      b1 += inv[i] * jr[1];
-      b2 += inv[i] * jr[2];
-      b3 += inv[i] * jr[3];
-    }
-    // B^T staging: Bt[b][a] = B[a][b]
-    S.Bt[(jb + 0) * 16 + a] = b0;
-    S.Bt[(jb + 1) * 16 + a] = b1;
-    S.Bt[(jb + 2) * 16 + a] = b2;
-    S.Bt[(jb + 3) * 16 + a] = b3;
-    if (a >= jb && a < jb + 4) {
-      const double dg = a == jb ? b0 : (a == jb + 1 ? b1 : (a == jb + 2 ? b2 : b3));
-      atomicAdd(&S.grad[c], dg);
-    }
-    wave_sync();
-    // sum_b B[a][b] B[b][a] with B[b][a] = Bt[a][b]... Bt holds B^T: Bt[x][y] = B[y][x]
-    // B[b][a] for b = jb..jb+3 is Bt[a][b]
-    const double* bt = &S.Bt[a * 16 + jb];
-    tsum += b0 * bt[0] + b1 * bt[1] + b2 * bt[2] + b3 * bt[3];
+    for (int v = 0; v < 4; ++v) tsum += A[v] * tile[lr * 17 + lg + 4 * v] + B[v] * tileb[lr * 17 + lg + 4 * v];
     wave_sync();
   }
   tsum = wave_sum_d(tsum);
@@ -745,7 +825,6 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
     o_[1] = (double)sgn;
     o_[2] = trAL - tsum;
   }
-  for (int c = lane; c < T; c += 64) o_[3 + c] = S.grad[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -901,9 +980,8 @@ size_t det_smem_bytes(int N, int* Np_out, int* cpi_out) {
 
 int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* env, double* out, hipStream_t s) {
   if (d.N <= SM_NP) {
-    const long ndet = (long)d.Bc * d.K;
-    hipLaunchKernelGGL(det_small_kernel, dim3((unsigned)((ndet + 3) / 4)), dim3(256), 0, s, d, Phi, ldphi, env, out,
-                       d.Rp > 1 ? 1 : 0);
+    hipLaunchKernelGGL(det_small_kernel, dim3((unsigned)xcd_grid(d.Bc, (d.K + 3) / 4)), dim3(256), 0, s, d, Phi, ldphi,
+                       env, out, d.Rp > 1 ? 1 : 0);
     return 0;
   }
   if (d.Rp > 1 && d.N <= 64) {
@@ -916,7 +994,7 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
       (void)hipFuncSetAttribute((const void*)det_mfma_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
       attr = true;
     }
-    const dim3 grid((unsigned)(d.Bc * d.K));
+    const dim3 grid((unsigned)xcd_grid(d.Bc, d.K));
     switch (np) {
       case 32: hipLaunchKernelGGL(det_mfma_kernel<32>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
       case 48: hipLaunchKernelGGL(det_mfma_kernel<48>, grid, dim3(256), smem, s, d, Phi, ldphi, env, out); return 0;
@@ -932,7 +1010,7 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
       (void)hipFuncSetAttribute((const void*)det_fwd_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024);
       attr_f = true;
     }
-    const dim3 grid((unsigned)((ndet + 1) / 2));
+    const dim3 grid((unsigned)xcd_grid(d.Bc, (d.K + 1) / 2));
     switch (np) {
       case 32: hipLaunchKernelGGL(det_fwd_kernel<32>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
       case 48: hipLaunchKernelGGL(det_fwd_kernel<48>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;

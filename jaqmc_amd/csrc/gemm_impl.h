@@ -51,19 +51,6 @@ __device__ inline void split3(float x, bf16* t, int ns) {
   }
 }
 
-// Workgroup -> (row tile, column block).  Workgroups are dispatched to the
-// eight XCDs round-robin (workgroup b runs on XCD b % 8, each XCD with its own
-// L2), so the column blocks of one row tile go to b, b + 8, b + 16, ...: they
-// run back to back on one XCD and the row tile's X panel is read from HBM once
-// instead of once per column block.  Row tiles past the end return at once.
-constexpr int kXcds = 8;
-__host__ __device__ inline void xcd_tile(int b, int ncb, int* rt, int* cb) {
-  const int x = b % kXcds, slot = b / kXcds;
-  *cb = slot % ncb;
-  *rt = (slot / ncb) * kXcds + x;
-}
-inline int xcd_grid(long nrt, int ncb) { return (int)((nrt + kXcds - 1) / kXcds) * kXcds * ncb; }
-
 // MBU: 16-row blocks per unit (Laplacian mode); UPT: units per tile; NBW:
 // 16-column blocks per wave.  BM = 16*MBU*UPT, BN = 64*NBW.
 // Waves per SIMD the register allocation must leave room for: three for tiles
