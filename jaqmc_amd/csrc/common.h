@@ -60,6 +60,30 @@ __host__ __device__ inline void xcd_tile(int b, int ncb, int* rt, int* cb) {
 }
 inline int xcd_grid(long nrt, int ncb) { return (int)((nrt + kXcds - 1) / kXcds) * kXcds * ncb; }
 
+// 64-bit DPP move (both halves with the same control)
+template <int CTRL>
+__device__ inline double dpp_d(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ inline double readlane_d(double x, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// Wavefront sum of a double, uniform result: DPP sums within each 16-lane row
+// (xor 1, xor 2, half-row mirror, row mirror), then the four row totals.
+__device__ inline double wave_sum_dpp_d(double x) {
+  x += dpp_d<0xB1>(x);
+  x += dpp_d<0x4E>(x);
+  x += dpp_d<0x141>(x);
+  x += dpp_d<0x140>(x);
+  return readlane_d(x, 0) + readlane_d(x, 16) + readlane_d(x, 32) + readlane_d(x, 48);
+}
+
 __device__ inline float wave_sum(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);

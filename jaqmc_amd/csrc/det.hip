@@ -317,68 +317,144 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
     }
     A[e] = a;
     E0[i * ES + o] = e0;
-    Inv[i * IS + o] = (i == o) ? 1.0 : 0.0;
   }
   __syncthreads();
+
+  // ---- Gauss-Jordan on [A | I] with the matrix in registers ----
+  // Thread (ti, tj) = (tid % 16, tid / 16) owns rows i = ti + 16a and columns
+  // j = tj + 16b of the NP x 2NP augmented matrix.  No rows are exchanged:
+  // the pivot for column p is the unused row q with the largest |A[q][p]|
+  // (smallest q on ties), and at the end row p of A^-1 is row q of the right
+  // half.  Per pivot: the owners of column p (16 lanes of one DPP row) have
+  // already reduced their candidates to (q, A[q][p]); barrier; the owners of
+  // row q and of column p publish them; barrier; every thread updates its
+  // elements and the owners of column p + 1 reduce the next candidates.
+  constexpr int RA = NP / 16, CB = NP / 8;
+  const int ti = tid & 15, tj = tid >> 4;
+  double x[RA][CB];
+#pragma unroll
+  for (int a = 0; a < RA; ++a)
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      const int i = ti + 16 * a, j = tj + 16 * b;
+      x[a][b] = j < NP ? A[i * NP + j] : (i == j - NP ? 1.0 : 0.0);
+    }
+  __syncthreads();  // A is scratch from here on
+  double* prow = A;             // pivot row, 2NP
+  double* pcol = A + 2 * NP;    // pivot column, NP
+  double* cval = A + 3 * NP;    // signed pivot value
+  int* cidx = (int*)(A + 3 * NP + 1);
+  int* perm = (int*)(A + 3 * NP + 2);  // perm[p] = pivot row of column p
+  unsigned used = 0;
+  int pos[RA];  // pivot index of the thread's rows
+  // candidates of a column held in x[.][b] by the 16 lanes of this DPP row
+  auto candidates = [&](int b) __attribute__((always_inline)) {
+    double v = -1.0, sv = 0.0;
+    int bi = NP;
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+      const double y = fabs(x[a][b]);
+      if (!((used >> a) & 1u) && (bi == NP || y > v)) {  // any unused row, even a non-finite one
+        v = y;
+        sv = x[a][b];
+        bi = ti + 16 * a;
+      }
+    }
+    // lanes without an unused row (bi == NP) take any offered candidate, so
+    // the result is a valid unused row whenever one exists
+    auto pick = [&](double ov, double osv, int oi) __attribute__((always_inline)) {
+      if (oi != NP && (bi == NP || ov > v || (ov == v && oi < bi))) {
+        v = ov;
+        sv = osv;
+        bi = oi;
+      }
+    };
+    pick(dpp_d<0xB1>(v), dpp_d<0xB1>(sv), __builtin_amdgcn_mov_dpp(bi, 0xB1, 0xF, 0xF, false));
+    pick(dpp_d<0x4E>(v), dpp_d<0x4E>(sv), __builtin_amdgcn_mov_dpp(bi, 0x4E, 0xF, 0xF, false));
+    pick(dpp_d<0x141>(v), dpp_d<0x141>(sv), __builtin_amdgcn_mov_dpp(bi, 0x141, 0xF, 0xF, false));
+    pick(dpp_d<0x140>(v), dpp_d<0x140>(sv), __builtin_amdgcn_mov_dpp(bi, 0x140, 0xF, 0xF, false));
+    if (ti == 0) {
+      *cval = sv;
+      *cidx = bi;
+    }
+  };
+  if (tj == 0) candidates(0);
   double logdet = 0.0;
   int sgn = 1;
-  for (int p = 0; p < NP; ++p) {
-    if (tid < 64) {
-      double best = -1.0;
-      int bi = p;
-      for (int i = p + tid; i < NP; i += 64) {
-        const double v = fabs(A[i * NP + p]);
-        if (v > best) {
-          best = v;
-          bi = i;
+#pragma unroll
+  for (int pb = 0; pb < RA; ++pb) {
+    for (int pr = 0; pr < 16; ++pr) {
+      const int p = 16 * pb + pr;
+      __syncthreads();
+      const int q = *cidx;
+      const double piv = *cval;
+      const double rp = 1.0 / piv;
+      if (tid == 0) {
+        logdet += log(fabs(piv));
+        if (piv < 0) sgn = -sgn;
+        perm[p] = q;
+      }
+      if (ti == (q & 15)) {
+#pragma unroll
+        for (int a = 0; a < RA; ++a)
+          if (a == (q >> 4)) {
+#pragma unroll
+            for (int b = 0; b < CB; ++b) prow[tj + 16 * b] = x[a][b];
+          }
+      }
+      if (tj == pr) {
+#pragma unroll
+        for (int a = 0; a < RA; ++a) pcol[ti + 16 * a] = x[a][pb];
+      }
+      __syncthreads();
+      double r[CB];
+#pragma unroll
+      for (int b = 0; b < CB; ++b) r[b] = prow[tj + 16 * b] * rp;
+#pragma unroll
+      for (int a = 0; a < RA; ++a) {
+        const int i = ti + 16 * a;
+        if (i == q) {
+#pragma unroll
+          for (int b = 0; b < CB; ++b) x[a][b] = r[b];
+          used |= 1u << a;
+          pos[a] = p;
+        } else {
+          const double f = pcol[i];
+#pragma unroll
+          for (int b = 0; b < CB; ++b) x[a][b] -= f * r[b];
         }
       }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o);
-        const int oi = __shfl_xor(bi, o);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
+      if (p + 1 < NP && tj == ((p + 1) & 15)) {
+        if (pr == 15) {
+          candidates(pb + 1 < RA ? pb + 1 : pb);  // p + 1 < NP implies pb + 1 < RA here
+        } else {
+          candidates(pb);
         }
       }
-      if (tid == 0) ipiv[0] = bi;
     }
-    __syncthreads();
-    const int q = ipiv[0];
-    if (q != p) {
-      for (int c = tid; c < NP; c += 256) {
-        double t = A[p * NP + c];
-        A[p * NP + c] = A[q * NP + c];
-        A[q * NP + c] = t;
-        t = Inv[p * IS + c];
-        Inv[p * IS + c] = Inv[q * IS + c];
-        Inv[q * IS + c] = t;
+  }
+  // row p of A^-1 = right half of row perm[p]
+#pragma unroll
+  for (int a = 0; a < RA; ++a)
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+      const int j = tj + 16 * b;
+      if (j >= NP) Inv[pos[a] * IS + (j - NP)] = x[a][b];
+    }
+  __syncthreads();
+  if (tid == 0) {
+    // sign of the row permutation: parity from its cycles
+    unsigned long long seen = 0ull;
+    for (int p = 0; p < NP; ++p) {
+      if ((seen >> p) & 1ull) continue;
+      int len = 0, c = p;
+      while (len < NP && c < NP && !((seen >> c) & 1ull)) {
+        seen |= 1ull << c;
+        c = perm[c];
+        ++len;
       }
-      sgn = -sgn;
+      if ((len & 1) == 0) sgn = -sgn;
     }
-    __syncthreads();
-    const double pv = A[p * NP + p];
-    logdet += log(fabs(pv));
-    if (pv < 0) sgn = -sgn;
-    const double rp = 1.0 / pv;
-    // elimination factors (column p of A) go to Inv's padding column NP
-    for (int c = tid; c < NP; c += 256) Inv[c * IS + NP] = (c == p) ? 0.0 : A[c * NP + p];
-    __syncthreads();
-    for (int c = tid; c < NP; c += 256) {
-      A[p * NP + c] *= rp;
-      Inv[p * IS + c] *= rp;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < NP * 2 * NP; idx += 256) {
-      const int i = idx / (2 * NP), c = idx % (2 * NP);
-      if (i == p) continue;
-      const double fi = Inv[i * IS + NP];
-      if (c < NP)
-        A[i * NP + c] -= fi * A[p * NP + c];
-      else
-        Inv[i * IS + (c - NP)] -= fi * Inv[p * IS + (c - NP)];
-    }
-    __syncthreads();
   }
   double* o_ = out + det_id * (3 + T);
 
@@ -580,30 +656,6 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
 // tr(B_c) from the diagonal lanes and tr(B_c^2) from a transposed read of the
 // tile in the wave's LDS slice.
 // ---------------------------------------------------------------------------
-
-// 64-bit DPP move (both halves with the same control)
-template <int CTRL>
-__device__ inline double dpp_d(double x) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-__device__ inline double readlane_d(double x, int l) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
-  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
-  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-// Wavefront sum of a double, uniform result: DPP sums within each 16-lane row
-// (xor 1, xor 2, half-row mirror, row mirror), then the four row totals.
-__device__ inline double wave_sum_dpp_d(double x) {
-  x += dpp_d<0xB1>(x);
-  x += dpp_d<0x4E>(x);
-  x += dpp_d<0x141>(x);
-  x += dpp_d<0x140>(x);
-  return readlane_d(x, 0) + readlane_d(x, 16) + readlane_d(x, 32) + readlane_d(x, 48);
-}
 
 constexpr int SM_NP = 16;
 constexpr int SM_PF = 4;  // coordinates in flight in the small kernel (even)
