@@ -253,7 +253,7 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
 
 // ---------------------------------------------------------------------------
 // Larger systems (16 < N <= 64): the Laplacian traces with FP64 MFMA.
-// Gauss-Jordan as in det_kernel (all four waves, LDS).  The coordinate loop
+// Gauss-Jordan with the augmented matrix in registers (see below).  The coordinate loop
 // then runs without workgroup barriers: wave q owns coordinates c = q, q+4, ...
 // and forms every 16x16 tile of B_c = A^-1 J_c in its accumulators with
 // v_mfma_f64_16x16x4f64.  The A operand (rows of A^-1) comes from LDS; the B
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const flo
 // values env[i][o] held in LDS.  The envelope-derivative part of J_c is nonzero
 // only in row c/3 and enters as the rank-1 update B_c += A^-1[:, c/3] u^T.
 // tr(B_c) is read from the diagonal tiles and tr(B_c^2) = sum_{PQ} <B_PQ,
-// B_QP^T> from a per-wave transpose of one tile at a time in LDS.
+// B_QP^T> from per-wave transposes in LDS, one column block of tiles at a time.
 // MFMA operand layout (16x16x4, f64): lane (r = lane%16, g = lane/16) supplies
 // A[r][4kk + g] and B[4kk + g][r]; result element v of the lane is row g + 4v,
 // column r (this differs from the f32 16x16x4 result, rows 4g + v; checked on
@@ -279,7 +279,8 @@ constexpr int det_mfma_env_stride() {
 }
 template <int NP>
 constexpr size_t det_mfma_smem() {
-  return (size_t)((NP * NP > 4 * 16 * DM_TS ? NP * NP : 4 * 16 * DM_TS) + NP * det_mfma_inv_stride<NP>() + 8) * 8 +
+  return (size_t)((NP * NP > 4 * (NP / 16) * 16 * DM_TS ? NP * NP : 4 * (NP / 16) * 16 * DM_TS) +
+                  NP * det_mfma_inv_stride<NP>() + 8) * 8 +
          (size_t)NP * det_mfma_env_stride<NP>() * 4 + 16;
 }
 
@@ -288,7 +289,8 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
                                                        const float* __restrict__ env, double* __restrict__ out) {
   constexpr int IS = det_mfma_inv_stride<NP>();
   constexpr int NT = NP / 16, KK = NP / 4;
-  constexpr int AREG = NP * NP > 4 * 16 * DM_TS ? NP * NP : 4 * 16 * DM_TS;
+  constexpr int TWS = NT * 16 * DM_TS;  // a wave's transpose tiles: one column block of B_c
+  constexpr int AREG = NP * NP > 4 * TWS ? NP * NP : 4 * TWS;
   extern __shared__ double sm[];
   double* A = sm;           // NP*NP for the elimination, then the wave transpose tiles
   double* Inv = A + AREG;   // NP*IS
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
 
   // ---- coordinate loop, no workgroup barriers ----
   const int lr = lane & 15, lg = lane >> 4;
-  double* tile = A + wave * 16 * DM_TS;
+  double* tile = A + wave * TWS;
   int urow[KK];  // unit * Rp + ROW_J0 for the lane's matrix row m = 4kk + lg, or -1 past N
   float jv[KK][NT];
 #pragma unroll
@@ -541,20 +543,24 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
       }
     tr = wave_sum_d(tr);
     if (lane == 0) o_[3 + c] = tr;
-    // tr(B_c^2): tile (P, Q) against the transpose of tile (Q, P)
+    // tr(B_c^2) = sum_PQ <B_PQ, B_QP^T>: for each P the tiles (Q, P), Q >= P,
+    // go through the wave's LDS slice together (one pair of wave barriers)
 #pragma unroll
-    for (int P = 0; P < NT; ++P)
+    for (int P = 0; P < NT; ++P) {
+#pragma unroll
+      for (int Q = P; Q < NT; ++Q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) tile[(Q - P) * 16 * DM_TS + (lg + 4 * v) * DM_TS + lr] = acc[Q][P][v];
+      wave_sync();
 #pragma unroll
       for (int Q = P; Q < NT; ++Q) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) tile[(lg + 4 * v) * DM_TS + lr] = acc[Q][P][v];
-        wave_sync();
         double s = 0.0;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) s += acc[P][Q][v] * tile[lr * DM_TS + lg + 4 * v];
+        for (int v = 0; v < 4; ++v) s += acc[P][Q][v] * tile[(Q - P) * 16 * DM_TS + lr * DM_TS + lg + 4 * v];
         tsum += (P == Q) ? s : 2.0 * s;
-        wave_sync();
       }
+      wave_sync();
+    }
   }
   tsum = wave_sum_d(tsum);
   if (lane == 0) red[4 + wave] = tsum;
