@@ -1,18 +1,17 @@
 // FP64 determinants of the orbital matrices and their forward Laplacian.
 //
-// One workgroup per (walker, determinant).  The orbital matrix
-// A[i][o] = phi[i][o] * env[i][o] (electron i, orbital o) is formed in FP64 in
-// LDS from the FP32 orbital-projection output and the envelope (product rule,
-// laplacian/primitives/core.py:494-500), inverted by Gauss-Jordan elimination
-// with partial pivoting, and log|det A| / sign taken from the pivots (the
-// values of jnp.linalg.slogdet, wavefunction/output/logdet.py:65).
+// The orbital matrix A[i][o] = phi[i][o] * env[i][o] (electron i, orbital o)
+// is formed in FP64 from the FP32 orbital-projection output and the envelope
+// (product rule, laplacian/primitives/core.py:494-500); log|det A| and its
+// sign are those of jnp.linalg.slogdet (wavefunction/output/logdet.py:65).
 //
 // Laplacian mode restates laplacian/primitives/slogdet.py:44-59:
 //   d logdet / dx_c = tr(A^-1 J_c)
 //   lap logdet      = tr(A^-1 L) - sum_c tr((A^-1 J_c)^2)
-// with B_c = A^-1 J_c computed in 4x4 register tiles.  A thread owns the tile
-// pair (P, Q) and (Q, P) of B_c, so tr(B_c^2) = sum_{P,Q} <B_PQ, B_QP^T> needs no
-// second pass; the diagonal tiles also give tr(B_c).
+// with B_c = A^-1 J_c on FP64 MFMA.  Three kernels: det_small_kernel
+// (N <= 16, one wavefront per determinant), det_mfma_kernel (16 < N <= 64,
+// Laplacian mode, one workgroup per determinant) and det_fwd_kernel
+// (16 < N <= 64, log|det| and sign only, one wavefront per determinant).
 
 #include <algorithm>
 
@@ -20,235 +19,9 @@
 
 namespace {
 
-constexpr int DET_THREADS = 256;
-
 // Orbital-matrix element (value, and optionally Jacobian row c or Laplacian).
 __device__ inline const float* env_ptr(const FermiDims& d, const float* env, int w, int i, int k, int o) {
   return env + ((((long)w * d.N + i) * d.K + k) * d.N + o) * 5;
-}
-
-__global__ __launch_bounds__(DET_THREADS) void det_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
-                                                          const float* __restrict__ env, double* __restrict__ out,
-                                                          int Np, int cpi, int lapmode) {
-  extern __shared__ double sm[];
-  const int N = d.N, T = 3 * N;
-  const int w = blockIdx.x / d.K, k = blockIdx.x % d.K;
-  const int tid = threadIdx.x;
-  double* A = sm;                // Np*Np   (later: inverse transposed)
-  double* Inv = sm + Np * Np;    // Np*Np   (later: J_c buffers)
-  double* red = sm + Np * Np + (long)max(Np * Np, cpi * Np * Np);  // scratch: 64 doubles + grad[T]
-  int* ipiv = (int*)(red + 64 + T);
-  const int Rp = d.Rp;
-
-  // ---- load A = phi * env, pad with identity ----
-  for (int idx = tid; idx < Np * Np; idx += DET_THREADS) {
-    const int i = idx / Np, o = idx % Np;
-    double a;
-    if (i < N && o < N) {
-      const int u = unit_idx(d, w, i);
-      const double phi = Phi[(long)u * Rp * ldphi + k * N + o];
-      a = phi * (double)env_ptr(d, env, w, i, k, o)[0];
-    } else {
-      a = (i == o) ? 1.0 : 0.0;
-    }
-    A[idx] = a;
-    Inv[idx] = (i == o) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-
-  // ---- Gauss-Jordan with partial pivoting on [A | I] ----
-  double logdet = 0.0;
-  int sgn = 1;
-  for (int p = 0; p < Np; ++p) {
-    if (tid < 64) {
-      // argmax |A[i][p]|, i >= p (one wave)
-      double best = -1.0;
-      int bi = p;
-      for (int i = p + tid; i < Np; i += 64) {
-        const double v = fabs(A[i * Np + p]);
-        if (v > best) {
-          best = v;
-          bi = i;
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o);
-        const int oi = __shfl_xor(bi, o);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if (tid == 0) ipiv[0] = bi;
-    }
-    __syncthreads();
-    const int q = ipiv[0];
-    if (q != p) {
-      for (int c = tid; c < Np; c += DET_THREADS) {
-        double t = A[p * Np + c];
-        A[p * Np + c] = A[q * Np + c];
-        A[q * Np + c] = t;
-        t = Inv[p * Np + c];
-        Inv[p * Np + c] = Inv[q * Np + c];
-        Inv[q * Np + c] = t;
-      }
-      sgn = -sgn;
-    }
-    __syncthreads();
-    const double pv = A[p * Np + p];
-    logdet += log(fabs(pv));
-    if (pv < 0) sgn = -sgn;
-    const double rp = 1.0 / pv;
-    // factors of column p (read before the row update)
-    for (int i = tid; i < Np; i += DET_THREADS) red[i] = (i == p) ? 0.0 : A[i * Np + p];
-    __syncthreads();
-    for (int c = tid; c < Np; c += DET_THREADS) {
-      A[p * Np + c] *= rp;
-      Inv[p * Np + c] *= rp;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < Np * 2 * Np; idx += DET_THREADS) {
-      const int i = idx / (2 * Np), c = idx % (2 * Np);
-      if (i == p) continue;
-      const double f = red[i];
-      if (c < Np)
-        A[i * Np + c] -= f * A[p * Np + c];
-      else
-        Inv[i * Np + (c - Np)] -= f * Inv[p * Np + (c - Np)];
-    }
-    __syncthreads();
-  }
-
-  double* o_ = out + (long)blockIdx.x * (3 + (lapmode ? T : 0));
-  if (!lapmode) {
-    if (tid == 0) {
-      o_[0] = logdet;
-      o_[1] = (double)sgn;
-    }
-    return;
-  }
-
-  // ---- InvT[i][o] = Inv[o][i] into A's space ----
-  for (int idx = tid; idx < Np * Np; idx += DET_THREADS) {
-    const int i = idx / Np, o = idx % Np;
-    A[idx] = Inv[o * Np + i];
-  }
-  for (int c = tid; c < T; c += DET_THREADS) red[64 + c] = 0.0;
-  __syncthreads();
-  double* InvT = A;
-  double* Jb = Inv;
-
-  // ---- L[i][o] (product rule with the envelope) and tr(A^-1 L) ----
-  double part = 0.0;
-  for (int idx = tid; idx < N * N; idx += DET_THREADS) {
-    const int i = idx / N, o = idx % N;
-    const int u = unit_idx(d, w, i);
-    const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
-    const float* e = env_ptr(d, env, w, i, k, o);
-    double L = (double)ph[ROW_LAP * ldphi] * e[0] + (double)ph[0] * e[1];
-    for (int ax = 0; ax < 3; ++ax) L += 2.0 * (double)ph[(ROW_J0 + 3 * i + ax) * ldphi] * e[2 + ax];
-    part += InvT[i * Np + o] * L;
-  }
-  part = wave_sum_d(part);
-  if ((tid & 63) == 0) red[tid >> 6] = part;
-  __syncthreads();
-  const double trAL = red[0] + red[1] + red[2] + red[3];
-  __syncthreads();
-
-  // ---- sum_c tr((A^-1 J_c)^2) and tr(A^-1 J_c) ----
-  const int nb = Np / 4;
-  const int npairs = nb * (nb + 1) / 2;
-  const int my_cc = tid / npairs;
-  int P = 0, Q = 0;
-  {
-    int rem = tid % npairs;
-    // enumerate P <= Q row by row
-    while (rem >= nb - P) {
-      rem -= nb - P;
-      ++P;
-    }
-    Q = P + rem;
-  }
-  double tsum = 0.0;
-  for (int c0 = 0; c0 < T; c0 += cpi) {
-    const int ncc = min(cpi, T - c0);
-    for (int idx = tid; idx < ncc * Np * Np; idx += DET_THREADS) {
-      const int cc = idx / (Np * Np);
-      const int io = idx % (Np * Np);
-      const int i = io / Np, o = io % Np;
-      double j = 0.0;
-      if (i < N && o < N) {
-        const int c = c0 + cc;
-        const int u = unit_idx(d, w, i);
-        const float* ph = Phi + (long)u * Rp * ldphi + k * N + o;
-        const float* e = env_ptr(d, env, w, i, k, o);
-        j = (double)ph[(ROW_J0 + c) * ldphi] * e[0];
-        if (c / 3 == i) j += (double)ph[0] * e[2 + c % 3];
-      }
-      Jb[idx] = j;
-    }
-    __syncthreads();
-    if (my_cc < ncc) {
-      const double* J = Jb + (long)my_cc * Np * Np;
-      double b1[4][4], b2[4][4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) b1[a][b] = b2[a][b] = 0.0;
-      for (int i = 0; i < Np; ++i) {
-        const double* ir = InvT + i * Np;
-        const double* jr = J + i * Np;
-        double ip[4], jq[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          ip[a] = ir[4 * P + a];
-          jq[a] = jr[4 * Q + a];
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) b1[a][b] += ip[a] * jq[b];
-        if (P != Q) {
-          double iq[4], jp[4];
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            iq[a] = ir[4 * Q + a];
-            jp[a] = jr[4 * P + a];
-          }
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) b2[a][b] += iq[a] * jp[b];
-        }
-      }
-      double t = 0.0;
-      if (P == Q) {
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) t += b1[a][b] * b1[b][a];
-        const double tr = b1[0][0] + b1[1][1] + b1[2][2] + b1[3][3];
-        atomicAdd(&red[64 + c0 + my_cc], tr);
-      } else {
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) t += b1[a][b] * b2[b][a];
-        t *= 2.0;
-      }
-      tsum += t;
-    }
-    __syncthreads();
-  }
-  tsum = wave_sum_d(tsum);
-  if ((tid & 63) == 0) red[tid >> 6] = tsum;
-  __syncthreads();
-  if (tid == 0) {
-    o_[0] = logdet;
-    o_[1] = (double)sgn;
-    o_[2] = trAL - (red[0] + red[1] + red[2] + red[3]);
-  }
-  for (int c = tid; c < T; c += DET_THREADS) o_[3 + c] = red[64 + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -607,16 +380,21 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
   int sgn = 1;
   const int ri = lane >> 3, cj = lane & 7;
   for (int p = 0; p < N; ++p) {
+    // largest |M[i][p]| over rows i >= p, smallest i on ties; lanes without a
+    // row offer none (index NP), so non-finite entries still yield a valid,
+    // wave-uniform pivot row
     double best = (p + lane < N) ? fabs(M[(p + lane) * S + p]) : -1.0;
-    int bi = p + lane;
+    int bi = (p + lane < N) ? p + lane : NP;
     for (int o = 32; o > 0; o >>= 1) {
       const double ob = __shfl_xor(best, o);
       const int oi = __shfl_xor(bi, o);
-      if (ob > best || (ob == best && oi < bi)) {
+      if (oi != NP && (bi == NP || ob > best || (ob == best && oi < bi))) {
         best = ob;
         bi = oi;
       }
     }
+    bi = __builtin_amdgcn_readlane(bi, 0);
+    if (bi >= N) bi = p;
     if (bi != p) {
       if (lane < N) {
         const double t = M[p * S + lane];
@@ -720,19 +498,20 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
   // pivot search over rows p..15 in lanes 0..15: DPP butterfly within the
   // first 16-lane row (largest |value|, smallest row on ties)
   auto pick = [&](double& v, int& bi, double ov, int oi) __attribute__((always_inline)) {
-    if (ov > v || (ov == v && oi < bi)) {
+    if (oi != SM_NP && (bi == SM_NP || ov > v || (ov == v && oi < bi))) {
       v = ov;
       bi = oi;
     }
   };
   for (int p = 0; p < SM_NP; ++p) {
     double v = (lane >= p && lane < SM_NP) ? fabs(S.M[lane * 32 + p]) : -1.0;
-    int bi = lane;
+    int bi = (lane >= p && lane < SM_NP) ? lane : SM_NP;  // SM_NP: no candidate
     pick(v, bi, dpp_d<0xB1>(v), __builtin_amdgcn_mov_dpp(bi, 0xB1, 0xF, 0xF, false));
     pick(v, bi, dpp_d<0x4E>(v), __builtin_amdgcn_mov_dpp(bi, 0x4E, 0xF, 0xF, false));
     pick(v, bi, dpp_d<0x141>(v), __builtin_amdgcn_mov_dpp(bi, 0x141, 0xF, 0xF, false));
     pick(v, bi, dpp_d<0x140>(v), __builtin_amdgcn_mov_dpp(bi, 0x140, 0xF, 0xF, false));
-    const int q = __builtin_amdgcn_readlane(bi, 0);
+    int q = __builtin_amdgcn_readlane(bi, 0);
+    if (q >= SM_NP) q = p;
     if (q != p) {
       if (lane < 32) {
         const double t0 = S.M[p * 32 + lane];
@@ -1021,21 +800,6 @@ __global__ __launch_bounds__(64) void combine_kernel(FermiDims d, CombineArgs a)
 
 }  // namespace
 
-size_t det_smem_bytes(int N, int* Np_out, int* cpi_out) {
-  const int Np = round_up(N, 4);
-  const int nb = Np / 4;
-  const int npairs = nb * (nb + 1) / 2;
-  int cpi = DET_THREADS / npairs;
-  if (cpi < 1) cpi = 1;
-  const long jb = 72 * 1024 / (8L * Np * Np);
-  if (cpi > jb) cpi = (int)(jb < 1 ? 1 : jb);
-  if (cpi > 3 * N) cpi = 3 * N;
-  *Np_out = Np;
-  *cpi_out = cpi;
-  const long T = 3L * N;
-  return (size_t)(8L * (Np * Np + (long)std::max(Np * Np, cpi * Np * Np) + 64 + T) + 16);
-}
-
 int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* env, double* out, hipStream_t s) {
   if (d.N <= SM_NP) {
     hipLaunchKernelGGL(det_small_kernel, dim3((unsigned)xcd_grid(d.Bc, (d.K + 3) / 4)), dim3(256), 0, s, d, Phi, ldphi,
@@ -1075,23 +839,7 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
       case 64: hipLaunchKernelGGL(det_fwd_kernel<64>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
     }
   }
-  int Np, cpi;
-  size_t smem = det_smem_bytes(d.N, &Np, &cpi);
-  if (d.Rp <= 1) {  // forward mode: no Jacobian buffers
-    cpi = 1;
-    smem = (size_t)(8L * (2L * Np * Np + 64 + 3L * d.N) + 16);
-  }
-  if (d.N > 64) return -1;
-  const int nb = Np / 4;
-  if (nb * (nb + 1) / 2 > DET_THREADS) return -1;
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipFuncSetAttribute((const void*)det_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
-  hipLaunchKernelGGL(det_kernel, dim3(d.Bc * d.K), dim3(DET_THREADS), smem, s, d, Phi, ldphi, env, out, Np, cpi,
-                     d.Rp > 1 ? 1 : 0);
-  return 0;
+  return -1;  // N > 64
 }
 
 int launch_combine(const FermiDims& d, const CombineArgs& a, hipStream_t s) {

@@ -9,7 +9,6 @@
 #include "prof.h"
 
 int gemm_unit_rows(int R);
-size_t det_smem_bytes(int N, int* Np_out, int* cpi_out);
 
 namespace {
 

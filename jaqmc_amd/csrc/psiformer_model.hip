@@ -20,7 +20,6 @@
 #include "model.h"
 #include "prof.h"
 
-size_t det_smem_bytes(int N, int* Np_out, int* cpi_out);
 
 namespace {
 struct Reader {

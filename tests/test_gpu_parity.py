@@ -284,6 +284,32 @@ def test_empty_batch_is_a_no_op(cuda):
     assert lp.numel() == 0 and out["e_loc"].numel() == 0 and acc.numel() == 0
 
 
+@pytest.mark.parametrize("case", ["n2_small", "h24"])
+def test_coincident_electrons_are_data(cuda, case):
+    """Two spin-up electrons at one point: that walker's orbital matrices are
+    singular and its pair features divide by r = 0.  The walker's results are
+    data (log|psi| far below its regular value or non-finite, no error) and
+    every other walker of the batch is bit-identical to the regular run
+    (estimator/base.py:44 treats non-finite E_L as data).  Covers the N <= 16
+    and 16 < N <= 64 determinant kernels."""
+    import torch
+
+    sysm, spec, p, x, _ = _setup(case, 6)
+    xs = x.copy()
+    xs[2, 1] = xs[2, 0]
+    dm = _model(sysm, spec, p)
+    lp, _ = dm.logpsi(torch.from_numpy(xs).to(cuda))
+    out = dm.local_energy(torch.from_numpy(xs).to(cuda))
+    lp0, _ = dm.logpsi(torch.from_numpy(x).to(cuda))
+    out0 = dm.local_energy(torch.from_numpy(x).to(cuda))
+    torch.cuda.synchronize()
+    keep = [0, 1, 3, 4, 5]
+    assert torch.equal(lp[keep], lp0[keep])
+    assert torch.equal(out["e_loc"][keep], out0["e_loc"][keep])
+    v = lp[2].item()
+    assert (not np.isfinite(v)) or v < lp0[2].item() - 5.0, (v, lp0[2].item())
+
+
 def test_psiformer_chunking_and_bf16x6(cuda):
     """Psiformer: chunked evaluation is bit-exact, and bf16x6 stays at FP32-level error."""
     import torch
