@@ -111,7 +111,7 @@ __global__ void ee_features_kernel(FermiDims d, int RP2, const float* __restrict
 // widths match), FermiLayers._update_double_stream (ferminet.py:57-63).
 // Thread = one tangent row of one pair (RP2 = 8 rows: value, Laplacian,
 // d/dx_a, d/dx_b); W staged in LDS and read as broadcast float4; the Local2
-// tanh rule sums J^2 over the pair's six Jacobian rows with xor-shuffles
+// tanh rule sums J^2 over the pair's six Jacobian rows with DPP lane moves
 // inside the 8-lane group of the pair.
 template <int RP2, int FIN, int FOUT>
 __global__ __launch_bounds__(256) void pair_layer_kernel(long npairs, const float* __restrict__ in,
@@ -126,36 +126,60 @@ __global__ __launch_bounds__(256) void pair_layer_kernel(long npairs, const floa
   const long nrows = npairs * RP2;
   const bool live = rowid < nrows;
   const int r = (int)(rowid % RP2);
-  float x[FIN];
   const float* xp = in + (live ? rowid : 0) * FIN;
-#pragma unroll
-  for (int k = 0; k < FIN; k += 4) *(float4*)&x[k] = *(const float4*)(xp + k);
   float acc[FOUT];
 #pragma unroll
   for (int f = 0; f < FOUT; ++f) acc[f] = (r == 0) ? bs[f] : 0.f;
+  if constexpr (FIN <= 4) {
+    const float4 xv = *(const float4*)xp;
+    const float xk[4] = {xv.x, xv.y, xv.z, xv.w};
 #pragma unroll 2
-  for (int k = 0; k < FIN; ++k) {
+    for (int k = 0; k < FIN; ++k)
 #pragma unroll
-    for (int f = 0; f < FOUT; f += 4) {
-      const float4 w = *(const float4*)&Ws[k * FOUT + f];
-      acc[f] += x[k] * w.x;
-      acc[f + 1] += x[k] * w.y;
-      acc[f + 2] += x[k] * w.z;
-      acc[f + 3] += x[k] * w.w;
+      for (int f = 0; f < FOUT; f += 4) {
+        const float4 w = *(const float4*)&Ws[k * FOUT + f];
+        acc[f] += xk[k] * w.x;
+        acc[f + 1] += xk[k] * w.y;
+        acc[f + 2] += xk[k] * w.z;
+        acc[f + 3] += xk[k] * w.w;
+      }
+  } else {
+    // the input row is consumed four features at a time (16-byte loads; the
+    // row stays in L1), so no private array is indexed with a runtime k
+#pragma unroll 2
+    for (int k4 = 0; k4 < FIN; k4 += 4) {
+      const float4 xv = *(const float4*)(xp + k4);
+      const float xk[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int f = 0; f < FOUT; f += 4) {
+          const float4 w = *(const float4*)&Ws[(k4 + kk) * FOUT + f];
+          acc[f] += xk[kk] * w.x;
+          acc[f + 1] += xk[kk] * w.y;
+          acc[f + 2] += xk[kk] * w.z;
+          acc[f + 3] += xk[kk] * w.w;
+        }
     }
   }
   float* yp = out + (live ? rowid : 0) * FOUT;
+  static_assert(RP2 == 1 || RP2 == 8, "pair rows");
+  const bool upper = (threadIdx.x & 4) != 0;
 #pragma unroll
   for (int f = 0; f < FOUT; ++f) {
-    // broadcast value and Laplacian rows of this pair; reduce J^2 over rows 2..7
+    // the pair's 8 rows are lanes 8g .. 8g+7: sum J^2 over rows 2..7 and
+    // broadcast the value row with DPP inside the group (quad_perm xor 1,
+    // xor 2, then the half-row mirror joins the two quads)
     float sq = (r >= 2) ? acc[f] * acc[f] : 0.f;
+    float v = acc[f];
     if (RP2 > 1) {
-      sq += __shfl_xor(sq, 1);
-      sq += __shfl_xor(sq, 2);
-      sq += __shfl_xor(sq, 4);
+      sq += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, sq), 0xB1, 0xF, 0xF, false));
+      sq += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, sq), 0x4E, 0xF, 0xF, false));
+      sq += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, sq), 0x141, 0xF, 0xF, false));
+      const float b = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x00, 0xF, 0xF, false));
+      const float m = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, b), 0x141, 0xF, 0xF, false));
+      v = upper ? m : b;
     }
-    const int lane0 = (threadIdx.x & 63) & ~(RP2 - 1);
-    const float v = RP2 > 1 ? __shfl(acc[f], lane0) : acc[f];
     const float t = tanhf(v);
     const float d1 = 1.f - t * t;
     float o;
@@ -165,7 +189,17 @@ __global__ __launch_bounds__(256) void pair_layer_kernel(long npairs, const floa
       o = d1 * acc[f] - 2.f * t * d1 * sq;
     else
       o = d1 * acc[f];
-    acc[f] = residual ? (o + x[f < FIN ? f : 0]) * 0.70710678118654752f : o;
+    acc[f] = o;
+  }
+  if (residual && FIN == FOUT) {
+#pragma unroll
+    for (int f = 0; f < FOUT; f += 4) {
+      const float4 xv = *(const float4*)(xp + f);
+      acc[f] = (acc[f] + xv.x) * 0.70710678118654752f;
+      acc[f + 1] = (acc[f + 1] + xv.y) * 0.70710678118654752f;
+      acc[f + 2] = (acc[f + 2] + xv.z) * 0.70710678118654752f;
+      acc[f + 3] = (acc[f + 3] + xv.w) * 0.70710678118654752f;
+    }
   }
   if (live) {
 #pragma unroll
