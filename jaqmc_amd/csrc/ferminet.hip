@@ -57,7 +57,9 @@ __global__ __launch_bounds__(256) void ae_features_kernel(FermiDims d, const flo
       if (c < 3 * d.N && c / 3 == i) {
         const int ax = c - 3 * i;
         o[0] = (ax == 0 ? g.x : (ax == 1 ? g.y : g.z)) / g.w;
-        o[1 + ax] = 1.f;
+        o[1] = ax == 0 ? 1.f : 0.f;  // static indices keep o in registers
+        o[2] = ax == 1 ? 1.f : 0.f;
+        o[3] = ax == 2 ? 1.f : 0.f;
       }
     }
     T* dst = Xu + (long)r * ldx + 4 * I;

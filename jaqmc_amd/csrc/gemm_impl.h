@@ -162,13 +162,11 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
 #pragma unroll
     for (int j = 0; j < NBW; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   // Laplacian mode: the accumulators start from the broadcast term
-  // G[walker, row-in-unit, col], so the epilogue needs no extra loads.  acc[mb][nb][v] is tile row mb*16 + 4*lg + v,
-  // column wcol + nb*16 + lr.  Units are whole (rows % RU == 0), so validity
-  // is per unit; columns past ncols read a clamped address and add nothing.
-  // (For tiles taller than 128 rows the loads are deferred to the epilogue,
-  // where they do not add to the K loop's register peak.)
-  constexpr bool G_AT_INIT = MB <= 8;
-  if constexpr (LAP && G_AT_INIT) {
+  // G[walker, row-in-unit, col], so the epilogue needs no extra loads.
+  // acc[mb][nb][v] is tile row mb*16 + 4*lg + v, column wcol + nb*16 + lr.
+  // Units are whole (rows % RU == 0), so validity is per unit; columns past
+  // ncols read a clamped address and add nothing.
+  if constexpr (LAP) {
     int ncl[NBW];
     bool cin[NBW];
 #pragma unroll
@@ -288,30 +286,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     ncol[nb] = col0 + wcol + nb * 16 + lr;
     cok[nb] = ncol[nb] < a.ncols;
   }
-  // pass 1 (Laplacian mode): broadcast term (tall tiles) and bias on the value rows
-  if constexpr (LAP && !G_AT_INIT) {
-    if (a.G) {
-#pragma unroll
-      for (int u = 0; u < UPT; ++u) {
-        const long unit = row0 / RU + u;
-        if ((unit + 1) * RU <= a.rows) {
-          const long w = a.wmaj ? unit / (a.n_up + a.n_dn) : walker_of_unit((int)unit, a.Bc, a.n_up, a.n_dn);
-          const float* gu = a.G + w * RU * a.ldg;
-#pragma unroll
-          for (int m = 0; m < MBU; ++m)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const float* gr = gu + (long)(m * 16 + 4 * lg + v) * a.ldg;
-#pragma unroll
-              for (int nb = 0; nb < NBW; ++nb) {
-                const float g = gr[cok[nb] ? ncol[nb] : a.ncols - 1];
-                acc[u * MBU + m][nb][v] += cok[nb] ? g : 0.f;
-              }
-            }
-        }
-      }
-    }
-  }
+  // pass 1 (Laplacian mode): bias on the value rows
   if (LAP && a.bias) {
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb) {
