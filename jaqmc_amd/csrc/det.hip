@@ -354,8 +354,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
 // ---------------------------------------------------------------------------
 template <int NP>
 __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
-                                                      const float* __restrict__ env, double* __restrict__ out,
-                                                      long ndet) {
+                                                      const float* __restrict__ env, double* __restrict__ out) {
   constexpr int S = NP + 1;  // odd row stride (doubles)
   constexpr int NB8 = NP / 8;
   extern __shared__ double smf[];
@@ -366,7 +365,6 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
   const int k = 2 * cb + wv;
   if (w >= d.Bc || k >= d.K) return;  // the wave's own determinant; nothing is shared across waves
   const long det = (long)w * d.K + k;
-  (void)ndet;
   double* M = smf + wv * NP * S;
   const int N = d.N;
 #pragma unroll 8
@@ -824,7 +822,6 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
     }
   }
   if (d.Rp <= 1 && d.N <= 64) {
-    const long ndet = (long)d.Bc * d.K;
     const int np = round_up(d.N, 16);
     const size_t smem = (size_t)2 * np * (np + 1) * sizeof(double);
     static bool attr_f = false;
@@ -834,9 +831,9 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
     }
     const dim3 grid((unsigned)xcd_grid(d.Bc, (d.K + 1) / 2));
     switch (np) {
-      case 32: hipLaunchKernelGGL(det_fwd_kernel<32>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
-      case 48: hipLaunchKernelGGL(det_fwd_kernel<48>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
-      case 64: hipLaunchKernelGGL(det_fwd_kernel<64>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out, ndet); return 0;
+      case 32: hipLaunchKernelGGL(det_fwd_kernel<32>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out); return 0;
+      case 48: hipLaunchKernelGGL(det_fwd_kernel<48>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out); return 0;
+      case 64: hipLaunchKernelGGL(det_fwd_kernel<64>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out); return 0;
     }
   }
   return -1;  // N > 64
