@@ -446,9 +446,10 @@ struct SmallDetSmem {
   double Tt[2 * SM_NP * 17];    // transpose tiles of B_c, B_c+1 (odd row stride)
 };
 
+template <bool LAPMODE>
 __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
-                                                        const float* __restrict__ env, double* __restrict__ out,
-                                                        int lapmode) {
+                                                        const float* __restrict__ env, double* __restrict__ out) {
+  constexpr bool lapmode = LAPMODE;
   __shared__ SmallDetSmem smem[4];
   const int N = d.N, T = 3 * N;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -501,6 +502,49 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
       bi = oi;
     }
   };
+  if constexpr (!lapmode) {
+    // forward mode needs only log|det| and the sign: LU with partial
+    // pivoting on the left 16x16 half.  A step reads column p, row p and the
+    // lane's own elements and writes only elements right of and below the
+    // pivot, so it needs a wave barrier after the row exchange and one after
+    // the update.
+    for (int p = 0; p < SM_NP; ++p) {
+      double v = (lane >= p && lane < SM_NP) ? fabs(S.M[lane * 32 + p]) : -1.0;
+      int bi = (lane >= p && lane < SM_NP) ? lane : SM_NP;
+      pick(v, bi, dpp_d<0xB1>(v), __builtin_amdgcn_mov_dpp(bi, 0xB1, 0xF, 0xF, false));
+      pick(v, bi, dpp_d<0x4E>(v), __builtin_amdgcn_mov_dpp(bi, 0x4E, 0xF, 0xF, false));
+      pick(v, bi, dpp_d<0x141>(v), __builtin_amdgcn_mov_dpp(bi, 0x141, 0xF, 0xF, false));
+      pick(v, bi, dpp_d<0x140>(v), __builtin_amdgcn_mov_dpp(bi, 0x140, 0xF, 0xF, false));
+      int q = __builtin_amdgcn_readlane(bi, 0);
+      if (q >= SM_NP) q = p;
+      if (q != p) {
+        if (lane < SM_NP) {
+          const double t0 = S.M[p * 32 + lane];
+          S.M[p * 32 + lane] = S.M[q * 32 + lane];
+          S.M[q * 32 + lane] = t0;
+        }
+        sgn = -sgn;
+      }
+      wave_sync();
+      const double pv = S.M[p * 32 + p];
+      logdet += log(fabs(pv));
+      if (pv < 0) sgn = -sgn;
+      const double rp = 1.0 / pv;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int e = lane + 64 * t, i = e >> 4, j = e & 15;
+        if (i > p && j > p) S.M[i * 32 + j] -= S.M[i * 32 + p] * rp * S.M[p * 32 + j];
+      }
+      wave_sync();
+    }
+    if (lane == 0) {
+      double* o_ = out + det_id * 3;
+      o_[0] = logdet;
+      o_[1] = (double)sgn;
+      o_[2] = 0.0;
+    }
+    return;
+  }
   for (int p = 0; p < SM_NP; ++p) {
     double v = (lane >= p && lane < SM_NP) ? fabs(S.M[lane * 32 + p]) : -1.0;
     int bi = (lane >= p && lane < SM_NP) ? lane : SM_NP;  // SM_NP: no candidate
@@ -539,14 +583,7 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
     }
     wave_sync();
   }
-  double* o_ = out + det_id * (3 + (lapmode ? T : 0));
-  if (!lapmode) {
-    if (lane == 0) {
-      o_[0] = logdet;
-      o_[1] = (double)sgn;
-    }
-    return;
-  }
+  double* o_ = out + det_id * (3 + T);
 
   // tr(A^-1 L): lane's 4 elements (i, o) pair with Inv[o][i]
   double part = 0.0;
@@ -800,8 +837,11 @@ __global__ __launch_bounds__(64) void combine_kernel(FermiDims d, CombineArgs a)
 
 int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* env, double* out, hipStream_t s) {
   if (d.N <= SM_NP) {
-    hipLaunchKernelGGL(det_small_kernel, dim3((unsigned)xcd_grid(d.Bc, (d.K + 3) / 4)), dim3(256), 0, s, d, Phi, ldphi,
-                       env, out, d.Rp > 1 ? 1 : 0);
+    const dim3 grid((unsigned)xcd_grid(d.Bc, (d.K + 3) / 4));
+    if (d.Rp > 1)
+      hipLaunchKernelGGL(det_small_kernel<true>, grid, dim3(256), 0, s, d, Phi, ldphi, env, out);
+    else
+      hipLaunchKernelGGL(det_small_kernel<false>, grid, dim3(256), 0, s, d, Phi, ldphi, env, out);
     return 0;
   }
   if (d.Rp > 1 && d.N <= 64) {
