@@ -154,39 +154,46 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
   }
 }
 
-// The same rule with the unit's Rp x D tile read from HBM once into LDS (row
-// stride D + 1 floats: the row-per-thread statistics passes and the
-// feature-per-thread output pass are both bank-conflict free), used when the
-// tile fits.
+// The same rule with the unit's Rp x D tile read from HBM once into LDS, used
+// when the tile fits.  Rows are D + 4 floats apart, so every LDS access is a
+// 16-byte access: the row-per-thread statistics passes read four features at
+// a time (a row's start moves by one 16-byte slot per thread: no bank
+// conflicts), and the output pass gives each thread four consecutive features
+// of every (256 / (D/4))-th row, written back as 16-byte stores.
 template <int D>
 __global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, const float* __restrict__ x,
                                                                 float* __restrict__ y, const float* __restrict__ scale,
                                                                 const float* __restrict__ bias, float eps) {
-  constexpr int SD = D + 1;
-  extern __shared__ float tile[];  // Rp x SD
-  __shared__ float d0s[D];
+  constexpr int SD = D + 4;
+  constexpr int F4 = D / 4;       // float4 per row
+  constexpr int RG = 256 / F4;    // row groups of the output pass
+  static_assert(256 % F4 == 0, "D must divide 1024");
+  extern __shared__ __attribute__((aligned(16))) float tile[];  // Rp x SD
+  __shared__ __attribute__((aligned(16))) float d0s[D];
   __shared__ float mu[256], js[256], red[8];
+  __shared__ __attribute__((aligned(16))) float crs[RG * D];  // partial cross sums per row group
   const long base = (long)blockIdx.x * Rp * D;
   const int tid = threadIdx.x;
   const float4* x4 = (const float4*)(x + base);
-  constexpr int Q4 = D / 4;
 #pragma unroll 8
-  for (int idx = tid; idx < Rp * Q4; idx += 256) {
-    const int r = idx / Q4, q = idx - r * Q4;
-    const float4 v = x4[idx];
-    float* t = tile + r * SD + 4 * q;
-    t[0] = v.x;
-    t[1] = v.y;
-    t[2] = v.z;
-    t[3] = v.w;
+  for (int idx = tid; idx < Rp * F4; idx += 256) {
+    const int r = idx / F4, q = idx - r * F4;
+    *(float4*)(tile + r * SD + 4 * q) = x4[idx];
   }
   __syncthreads();
   float m = 0.f;
   if (tid < Rp) {
-    const float* row = tile + tid * SD;
+    const float4* row = (const float4*)(tile + tid * SD);
+    float m4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 8
-    for (int f = 0; f < D; ++f) m += row[f];
-    m *= 1.f / D;
+    for (int q = 0; q < F4; ++q) {
+      const float4 v = row[q];
+      m4[0] += v.x;
+      m4[1] += v.y;
+      m4[2] += v.z;
+      m4[3] += v.w;
+    }
+    m = ((m4[0] + m4[1]) + (m4[2] + m4[3])) * (1.f / D);
     mu[tid] = m;
   }
   __syncthreads();
@@ -194,13 +201,24 @@ __global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, c
   __syncthreads();
   float dot = 0.f, sq = 0.f;
   if (tid < Rp) {
-    const float* row = tile + tid * SD;
+    const float4* row = (const float4*)(tile + tid * SD);
+    const float4* d04 = (const float4*)d0s;
+    float dt[4] = {0.f, 0.f, 0.f, 0.f}, s4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 8
-    for (int f = 0; f < D; ++f) {
-      const float a0 = row[f] - m;
-      dot += a0 * d0s[f];
-      sq += a0 * a0;
+    for (int q = 0; q < F4; ++q) {
+      const float4 v = row[q], d = d04[q];
+      const float a0 = v.x - m, a1 = v.y - m, a2 = v.z - m, a3 = v.w - m;
+      dt[0] += a0 * d.x;
+      dt[1] += a1 * d.y;
+      dt[2] += a2 * d.z;
+      dt[3] += a3 * d.w;
+      s4[0] += a0 * a0;
+      s4[1] += a1 * a1;
+      s4[2] += a2 * a2;
+      s4[3] += a3 * a3;
     }
+    dot = (dt[0] + dt[1]) + (dt[2] + dt[3]);
+    sq = (s4[0] + s4[1]) + (s4[2] + s4[3]);
   }
   const bool jrow = tid >= ROW_J0 && tid < R;
   float p_lap = jrow ? sq : 0.f;
@@ -223,21 +241,41 @@ __global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, c
   __syncthreads();
   const float laps = s1 * lapv + s2 * (red[0] + red[1] + red[2] + red[3]);
   float* yu = y + base;
-  for (int f = tid; f < D; f += 256) {
-    const float g = scale[f], d0 = d0s[f];
-    float cross = 0.f;
-#pragma unroll 8
-    for (int r = ROW_J0; r < R; ++r) {
-      const float dr = tile[r * SD + f] - mu[r];
-      cross += dr * js[r];
-      yu[(long)r * D + f] = g * (dr * sv + d0 * js[r]);
+  const int f4 = tid % F4, ro = tid / F4, f = 4 * f4;
+  const float4 dd = *(const float4*)&d0s[f];
+  const float g[4] = {scale[f], scale[f + 1], scale[f + 2], scale[f + 3]}, d0[4] = {dd.x, dd.y, dd.z, dd.w};
+  float cr[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int r = ROW_J0 + ro; r < R; r += RG) {
+    const float4 t = *(const float4*)(tile + r * SD + f);
+    const float mr = mu[r], jr = js[r];
+    const float dr[4] = {t.x - mr, t.y - mr, t.z - mr, t.w - mr};
+    float o[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      cr[q] += dr[q] * jr;
+      o[q] = g[q] * (dr[q] * sv + d0[q] * jr);
     }
-    for (int r = R > ROW_J0 ? R : ROW_J0; r < Rp; ++r) yu[(long)r * D + f] = 0.f;
-    yu[f] = g * d0 * sv + bias[f];
-    if (Rp > 1) {
-      const float dl = tile[SD + f] - mu[1];
-      yu[D + f] = g * (dl * sv + d0 * laps + 2.f * cross);
+    *(float4*)(yu + (long)r * D + f) = float4{o[0], o[1], o[2], o[3]};
+  }
+  for (int r = (R > ROW_J0 ? R : ROW_J0) + ro; r < Rp; r += RG)
+    *(float4*)(yu + (long)r * D + f) = float4{0.f, 0.f, 0.f, 0.f};
+  *(float4*)&crs[ro * D + f] = float4{cr[0], cr[1], cr[2], cr[3]};
+  __syncthreads();
+  if (ro == 0) {
+    const float bb[4] = {bias[f], bias[f + 1], bias[f + 2], bias[f + 3]};
+    float ov[4], ol[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float cross = 0.f;
+#pragma unroll
+      for (int gq = 0; gq < RG; ++gq) cross += crs[gq * D + f + q];
+      ov[q] = g[q] * d0[q] * sv + bb[q];
+      const float dl = tile[SD + f + q] - mu[1];
+      ol[q] = g[q] * (dl * sv + d0[q] * laps + 2.f * cross);
     }
+    *(float4*)(yu + f) = float4{ov[0], ov[1], ov[2], ov[3]};
+    if (Rp > 1) *(float4*)(yu + D + f) = float4{ol[0], ol[1], ol[2], ol[3]};
   }
 }
 
@@ -357,8 +395,8 @@ int launch_psi_features(const FermiDims& d, const float* elec, const float* atom
 template <int D>
 bool launch_ln_lds(int units, int Rp, int R, const float* x, float* y, const float* scale, const float* bias,
                    float eps, hipStream_t s) {
-  const size_t smem = (size_t)Rp * (D + 1) * sizeof(float);
-  const size_t fixed = (D + 2 * 256 + 8) * sizeof(float);
+  const size_t smem = (size_t)Rp * (D + 4) * sizeof(float);
+  const size_t fixed = (D + 2 * 256 + 8 + 1024) * sizeof(float);
   if (smem + fixed > 160 * 1024) return false;
   static bool attr = false;
   if (!attr) {
