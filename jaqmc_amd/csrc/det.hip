@@ -545,41 +545,75 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
     }
     return;
   }
-  for (int p = 0; p < SM_NP; ++p) {
-    double v = (lane >= p && lane < SM_NP) ? fabs(S.M[lane * 32 + p]) : -1.0;
-    int bi = (lane >= p && lane < SM_NP) ? lane : SM_NP;  // SM_NP: no candidate
-    pick(v, bi, dpp_d<0xB1>(v), __builtin_amdgcn_mov_dpp(bi, 0xB1, 0xF, 0xF, false));
-    pick(v, bi, dpp_d<0x4E>(v), __builtin_amdgcn_mov_dpp(bi, 0x4E, 0xF, 0xF, false));
-    pick(v, bi, dpp_d<0x141>(v), __builtin_amdgcn_mov_dpp(bi, 0x141, 0xF, 0xF, false));
-    pick(v, bi, dpp_d<0x140>(v), __builtin_amdgcn_mov_dpp(bi, 0x140, 0xF, 0xF, false));
-    int q = __builtin_amdgcn_readlane(bi, 0);
-    if (q >= SM_NP) q = p;
-    if (q != p) {
-      if (lane < 32) {
-        const double t0 = S.M[p * 32 + lane];
-        S.M[p * 32 + lane] = S.M[q * 32 + lane];
-        S.M[q * 32 + lane] = t0;
+  // Gauss-Jordan on [A | I] with the matrix in registers and no row
+  // exchanges: lane (ti, tj) = (lane % 16, lane / 16) owns row ti, columns
+  // tj + 4b (b = 0..7).  The pivot of column p is the unused row with the
+  // largest |entry| (smallest row on ties), reduced with DPP inside the
+  // 16-lane row that holds column p; the pivot row and column are published
+  // through LDS; at the end row p of A^-1 is the right half of the pivot row
+  // of column p, and the parity of that row permutation enters the sign.
+  {
+    const int ti = lane & 15, tj = lane >> 4;
+    double x[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) x[b] = S.M[ti * 32 + tj + 4 * b];
+    bool used = false;
+    int pos = 0;
+    double* prow = S.Tt;       // 32 doubles
+    double* pcol = S.Tt + 32;  // 16 doubles
+    int* perm = (int*)(S.Tt + 48);
+    for (int p = 0; p < SM_NP; ++p) {
+      const int pb = p >> 2, pr = p & 3;
+      double xp = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (b == pb) xp = x[b];
+      double v = used ? -1.0 : fabs(xp);
+      int bi = used ? SM_NP : ti;
+      pick(v, bi, dpp_d<0xB1>(v), __builtin_amdgcn_mov_dpp(bi, 0xB1, 0xF, 0xF, false));
+      pick(v, bi, dpp_d<0x4E>(v), __builtin_amdgcn_mov_dpp(bi, 0x4E, 0xF, 0xF, false));
+      pick(v, bi, dpp_d<0x141>(v), __builtin_amdgcn_mov_dpp(bi, 0x141, 0xF, 0xF, false));
+      pick(v, bi, dpp_d<0x140>(v), __builtin_amdgcn_mov_dpp(bi, 0x140, 0xF, 0xF, false));
+      int q = __builtin_amdgcn_readlane(bi, 16 * pr);
+      if (q >= SM_NP) q = 0;  // unreachable: an unused row always remains
+      const double piv = readlane_d(xp, 16 * pr + q);
+      logdet += log(fabs(piv));
+      if (piv < 0) sgn = -sgn;
+      if (lane == 0) perm[p] = q;
+      if (ti == q) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) prow[tj + 4 * b] = x[b];
       }
-      sgn = -sgn;
-    }
-    wave_sync();
-    const double pv = S.M[p * 32 + p];
-    logdet += log(fabs(pv));
-    if (pv < 0) sgn = -sgn;
-    // factors of column p for the 8 elements this lane updates (rows 2*(lane/16).., see below)
-    double f[8];
+      if (tj == pr) pcol[ti] = xp;
+      wave_sync();
+      const double rp = 1.0 / piv;
+      const double f = ti == q ? 0.0 : pcol[ti];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int e = lane + 64 * t, i = e >> 5;
-      f[t] = S.M[i * 32 + p];
+      for (int b = 0; b < 8; ++b) {
+        const double r = prow[tj + 4 * b] * rp;
+        x[b] = ti == q ? r : x[b] - f * r;
+      }
+      if (ti == q) {
+        used = true;
+        pos = p;
+      }
+      wave_sync();
     }
-    wave_sync();
-    if (lane < 32) S.M[p * 32 + lane] /= pv;
-    wave_sync();
+    // row pos of A^-1 = right half of this lane's row
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int e = lane + 64 * t, i = e >> 5, j = e & 31;
-      if (i != p) S.M[i * 32 + j] -= f[t] * S.M[p * 32 + j];
+    for (int b = 4; b < 8; ++b) S.M[pos * 32 + tj + 4 * b] = x[b];
+    if (lane == 0) {
+      int seen = 0;
+      for (int p0 = 0; p0 < SM_NP; ++p0) {
+        if ((seen >> p0) & 1) continue;
+        int len = 0, c = p0;
+        while (len < SM_NP && c < SM_NP && !((seen >> c) & 1)) {
+          seen |= 1 << c;
+          c = perm[c];
+          ++len;
+        }
+        if ((len & 1) == 0) sgn = -sgn;
+      }
     }
     wave_sync();
   }
