@@ -14,6 +14,7 @@
 // (16 < N <= 64, log|det| and sign only, one wavefront per determinant).
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -445,12 +446,17 @@ struct SmallDetSmem {
   double M[SM_NP * 2 * SM_NP];  // augmented matrix, later Inv in columns 16..31
   double Tt[2 * SM_NP * 17];    // transpose tiles of B_c, B_c+1 (odd row stride)
 };
+struct SmallFwdSmem {           // forward mode: the matrix and the pivot exchange area
+  double M[SM_NP * 2 * SM_NP];
+  double Tt[48];
+};
 
 template <bool LAPMODE>
 __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
                                                         const float* __restrict__ env, double* __restrict__ out) {
   constexpr bool lapmode = LAPMODE;
-  __shared__ SmallDetSmem smem[4];
+  using Smem = typename std::conditional<LAPMODE, SmallDetSmem, SmallFwdSmem>::type;
+  __shared__ Smem smem[4];
   const int N = d.N, T = 3 * N;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // determinants 4cb .. 4cb + 3 of walker w; a walker's blocks share an XCD
@@ -459,7 +465,7 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
   const int k = 4 * cb + wave;
   if (w >= d.Bc || k >= d.K) return;  // whole wave exits together
   const long det_id = (long)w * d.K + k;
-  SmallDetSmem& S = smem[wave];
+  Smem& S = smem[wave];
   const int Rp = d.Rp;
 
   // per-lane elements e = lane + 64 t, t = 0..3: (i, o) = (e / 16, e % 16)
@@ -503,41 +509,67 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
     }
   };
   if constexpr (!lapmode) {
-    // forward mode needs only log|det| and the sign: LU with partial
-    // pivoting on the left 16x16 half.  A step reads column p, row p and the
-    // lane's own elements and writes only elements right of and below the
-    // pivot, so it needs a wave barrier after the row exchange and one after
-    // the update.
+    // forward mode needs only log|det| and the sign: LU without row
+    // exchanges with the 16x16 block in registers.  Lane (ti, tj) =
+    // (lane % 16, lane / 16) owns row ti, columns tj + 4b (b = 0..3); the
+    // pivot of column p is the unused row with the largest |entry| (DPP
+    // inside the 16-lane row that holds column p); the pivot row and column go
+    // through LDS and the rows not yet used are updated right of column p.
+    // The parity of the row order enters the sign.
+    const int ti = lane & 15, tj = lane >> 4;
+    double x[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) x[b] = S.M[ti * 32 + tj + 4 * b];
+    bool used = false;
+    double* prow = S.Tt;       // 16 doubles
+    double* pcol = S.Tt + 16;  // 16 doubles
+    int* perm = (int*)(S.Tt + 32);
     for (int p = 0; p < SM_NP; ++p) {
-      double v = (lane >= p && lane < SM_NP) ? fabs(S.M[lane * 32 + p]) : -1.0;
-      int bi = (lane >= p && lane < SM_NP) ? lane : SM_NP;
+      const int pb = p >> 2, pr = p & 3;
+      double xp = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        if (b == pb) xp = x[b];
+      double v = used ? -1.0 : fabs(xp);
+      int bi = used ? SM_NP : ti;
       pick(v, bi, dpp_d<0xB1>(v), __builtin_amdgcn_mov_dpp(bi, 0xB1, 0xF, 0xF, false));
       pick(v, bi, dpp_d<0x4E>(v), __builtin_amdgcn_mov_dpp(bi, 0x4E, 0xF, 0xF, false));
       pick(v, bi, dpp_d<0x141>(v), __builtin_amdgcn_mov_dpp(bi, 0x141, 0xF, 0xF, false));
       pick(v, bi, dpp_d<0x140>(v), __builtin_amdgcn_mov_dpp(bi, 0x140, 0xF, 0xF, false));
-      int q = __builtin_amdgcn_readlane(bi, 0);
-      if (q >= SM_NP) q = p;
-      if (q != p) {
-        if (lane < SM_NP) {
-          const double t0 = S.M[p * 32 + lane];
-          S.M[p * 32 + lane] = S.M[q * 32 + lane];
-          S.M[q * 32 + lane] = t0;
-        }
-        sgn = -sgn;
-      }
-      wave_sync();
-      const double pv = S.M[p * 32 + p];
-      logdet += log(fabs(pv));
-      if (pv < 0) sgn = -sgn;
-      const double rp = 1.0 / pv;
+      int q = __builtin_amdgcn_readlane(bi, 16 * pr);
+      if (q >= SM_NP) q = 0;  // unreachable: an unused row always remains
+      const double piv = readlane_d(xp, 16 * pr + q);
+      logdet += log(fabs(piv));
+      if (piv < 0) sgn = -sgn;
+      if (lane == 0) perm[p] = q;
+      if (ti == q) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int e = lane + 64 * t, i = e >> 4, j = e & 15;
-        if (i > p && j > p) S.M[i * 32 + j] -= S.M[i * 32 + p] * rp * S.M[p * 32 + j];
+        for (int b = 0; b < 4; ++b) prow[tj + 4 * b] = x[b];
+      }
+      if (tj == pr) pcol[ti] = xp;
+      wave_sync();
+      if (ti == q) {
+        used = true;
+      } else if (!used) {
+        const double f = pcol[ti] / piv;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (tj + 4 * b > p) x[b] -= f * prow[tj + 4 * b];
       }
       wave_sync();
     }
     if (lane == 0) {
+      int seen = 0;
+      for (int p0 = 0; p0 < SM_NP; ++p0) {
+        if ((seen >> p0) & 1) continue;
+        int len = 0, c = p0;
+        while (len < SM_NP && c < SM_NP && !((seen >> c) & 1)) {
+          seen |= 1 << c;
+          c = perm[c];
+          ++len;
+        }
+        if ((len & 1) == 0) sgn = -sgn;
+      }
       double* o_ = out + det_id * 3;
       o_[0] = logdet;
       o_[1] = (double)sgn;

@@ -492,10 +492,16 @@ int dispatch(const GemmArgs& a, hipStream_t s) {
   const bool wide = a.ncols > 128;
   if (a.R <= 1) {
     // two 128-column blocks share their X panel through the XCD's L2, which
-    // beats one 256-column block at a third of the occupancy
+    // beats one 256-column block at a third of the occupancy.  Launches of
+    // fewer than two rounds of 128-row tiles over the CUs (the Metropolis
+    // forward pass, the per-walker broadcast GEMMs there) take 64-row tiles,
+    // so the last round is not a mostly idle tail.
     const bool wide_tile = g_gemm_variant > 0 ? (g_gemm_variant % 10) == 4 : false;
+    const long tiles128 = (long)ceil_div(a.rows, 128) * ceil_div(a.ncols, 128) * (a.nbatch > 1 ? a.nbatch : 1);
     if (wide_tile)
       launch_one<8, 1, 4, PREC, false>(a, s);
+    else if (tiles128 < 2L * 3 * 256)
+      launch_one<4, 1, 2, PREC, false>(a, s);
     else
       launch_one<8, 1, 2, PREC, false>(a, s);
     return 0;
