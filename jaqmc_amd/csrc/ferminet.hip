@@ -449,6 +449,33 @@ inline int grid_for(long total) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Forward mode (one row per pair and per unit): thread per output column of
+// a unit, X[unit(w, i)][col0 + s*f2 + f] = mean_{j in s} H2[w][j][i][f]
+// (summed in the same j order as the Laplacian-mode kernel), columns past
+// nch*f2 zero.  No staging: a unit's work is a few hundred floats.
+template <typename T>
+__global__ __launch_bounds__(256) void pair_means_fwd_kernel(FermiDims d, const float* __restrict__ H2, int f2,
+                                                             T* __restrict__ X, long ldx, int col0) {
+  const int ncol = (int)(ldx - col0);
+  const long total = (long)d.Bc * d.N * ncol;
+  for (long idx = blockIdx.x * 256L + threadIdx.x; idx < total; idx += (long)gridDim.x * 256) {
+    const int cc = (int)(idx % ncol);
+    const long wi = idx / ncol;
+    const int w = (int)(wi / d.N), i = (int)(wi % d.N);
+    float out = 0.f;
+    if (cc < d.nch * f2) {
+      const int s = cc / f2, f = cc - s * f2;
+      const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
+      const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
+      const float* p = H2 + (((long)w * d.N + j0) * d.N + i) * f2 + f;
+      float acc = 0.f;
+      for (int j = 0; j < ns; ++j) acc += p[(long)j * d.N * f2];
+      out = acc / ns;
+    }
+    X[(long)unit_of(w, i, d.Bc, d.n_up, d.n_dn) * ldx + col0 + cc] = from_f<T>(out);
+  }
+}
+
 }  // namespace
 
 int launch_ferminet_features(const FermiDims& d, const float* elec, const float* atoms, void* X, long ldx,
@@ -504,6 +531,15 @@ int launch_pair_layer(const FermiDims& d, const float* H2in, int fin, float* H2o
 int launch_pair_means(const FermiDims& d, const float* H2, int f2, void* X, long ldx, int col0, bool x_bf16,
                       hipStream_t s) {
   const int RP2 = d.Rp > 1 ? 8 : 1;
+  if (RP2 == 1) {
+    const long total = (long)d.Bc * d.N * (ldx - col0);
+    if (total <= 0) return 0;
+    if (x_bf16)
+      hipLaunchKernelGGL(pair_means_fwd_kernel<bf16>, grid_for(total), 256, 0, s, d, H2, f2, (bf16*)X, ldx, col0);
+    else
+      hipLaunchKernelGGL(pair_means_fwd_kernel<float>, grid_for(total), 256, 0, s, d, H2, f2, (float*)X, ldx, col0);
+    return 0;
+  }
   const size_t smem = ((size_t)d.N * RP2 * f2 + 2 * 5 * f2) * 4;
   if (smem > 64 * 1024) return -1;
   const dim3 grid((unsigned)((long)d.Bc * d.N));
