@@ -20,6 +20,23 @@
 
 namespace {
 
+// log|prod_p v_p| over the pivots without one log per factor: the running
+// product is kept as a mantissa in [0.5, 1) and a binary exponent (both
+// factors split with frexp, so it neither overflows nor underflows), and one
+// log is taken at the end.  A zero pivot gives -inf, a NaN or inf pivot
+// propagates, as the sum of logs would.
+struct LogAbsProduct {
+  double m = 1.0;
+  int e = 0;
+  __device__ void mul(double v) {
+    int ev, em;
+    const double mv = frexp(fabs(v), &ev);
+    m = frexp(m * mv, &em);
+    e += ev + em;
+  }
+  __device__ double log_abs() const { return log(m) + (double)e * 0.69314718055994530942; }
+};
+
 // Orbital-matrix element (value, and optionally Jacobian row c or Laplacian).
 __device__ inline const float* env_ptr(const FermiDims& d, const float* env, int w, int i, int k, int o) {
   return env + ((((long)w * d.N + i) * d.K + k) * d.N + o) * 5;
@@ -155,7 +172,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
     }
   };
   if (tj == 0) candidates(0);
-  double logdet = 0.0;
+  LogAbsProduct logdet;
   int sgn = 1;
 #pragma unroll
   for (int pb = 0; pb < RA; ++pb) {
@@ -166,7 +183,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
       const double piv = *cval;
       const double rp = 1.0 / piv;
       if (tid == 0) {
-        logdet += log(fabs(piv));
+        logdet.mul(piv);
         if (piv < 0) sgn = -sgn;
         perm[p] = q;
       }
@@ -340,7 +357,7 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
   if (lane == 0) red[4 + wave] = tsum;
   __syncthreads();
   if (tid == 0) {
-    o_[0] = logdet;
+    o_[0] = logdet.log_abs();
     o_[1] = (double)sgn;
     o_[2] = trAL - (red[4] + red[5] + red[6] + red[7]);
   }
@@ -375,7 +392,7 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
                    (double)env_ptr(d, env, w, i, k, o)[0];
   }
   wave_sync();
-  double logdet = 0.0;
+  LogAbsProduct logdet;
   int sgn = 1;
   const int ri = lane >> 3, cj = lane & 7;
   for (int p = 0; p < N; ++p) {
@@ -404,7 +421,7 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
     }
     wave_sync();
     const double pv = M[p * S + p];
-    logdet += log(fabs(pv));
+    logdet.mul(pv);
     if (pv < 0) sgn = -sgn;
     const double rp = 1.0 / pv;
     double prow[NB8];
@@ -424,7 +441,7 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
     wave_sync();
   }
   if (lane == 0) {
-    out[det * 3] = logdet;
+    out[det * 3] = logdet.log_abs();
     out[det * 3 + 1] = (double)sgn;
     out[det * 3 + 2] = 0.0;
   }
@@ -498,7 +515,7 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
   }
   wave_sync();
 
-  double logdet = 0.0;
+  LogAbsProduct logdet;
   int sgn = 1;
   // pivot search over rows p..15 in lanes 0..15: DPP butterfly within the
   // first 16-lane row (largest |value|, smallest row on ties)
@@ -539,7 +556,7 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
       int q = __builtin_amdgcn_readlane(bi, 16 * pr);
       if (q >= SM_NP) q = 0;  // unreachable: an unused row always remains
       const double piv = readlane_d(xp, 16 * pr + q);
-      logdet += log(fabs(piv));
+      logdet.mul(piv);
       if (piv < 0) sgn = -sgn;
       if (lane == 0) perm[p] = q;
       if (ti == q) {
@@ -548,10 +565,11 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
       }
       if (tj == pr) pcol[ti] = xp;
       wave_sync();
+      const double rp = 1.0 / piv;
       if (ti == q) {
         used = true;
       } else if (!used) {
-        const double f = pcol[ti] / piv;
+        const double f = pcol[ti] * rp;
 #pragma unroll
         for (int b = 0; b < 4; ++b)
           if (tj + 4 * b > p) x[b] -= f * prow[tj + 4 * b];
@@ -571,7 +589,7 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
         if ((len & 1) == 0) sgn = -sgn;
       }
       double* o_ = out + det_id * 3;
-      o_[0] = logdet;
+      o_[0] = logdet.log_abs();
       o_[1] = (double)sgn;
       o_[2] = 0.0;
     }
@@ -609,7 +627,7 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
       int q = __builtin_amdgcn_readlane(bi, 16 * pr);
       if (q >= SM_NP) q = 0;  // unreachable: an unused row always remains
       const double piv = readlane_d(xp, 16 * pr + q);
-      logdet += log(fabs(piv));
+      logdet.mul(piv);
       if (piv < 0) sgn = -sgn;
       if (lane == 0) perm[p] = q;
       if (ti == q) {
@@ -759,7 +777,7 @@ __global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float
   }
   tsum = wave_sum_d(tsum);
   if (lane == 0) {
-    o_[0] = logdet;
+    o_[0] = logdet.log_abs();
     o_[1] = (double)sgn;
     o_[2] = trAL - tsum;
   }
