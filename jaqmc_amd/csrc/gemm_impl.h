@@ -201,9 +201,20 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     }
   }
 
-  gload(0);
-  bload(0);
-  for (int k0 = 0; k0 < a.K; k0 += KC) {
+  // When the residual is a slice of this GEMM's own input (X[row][n] for
+  // output column n), the K loop is rotated to end on the chunks that hold
+  // the tile's columns col0 .. col0+BN-1: the epilogue's residual reads then
+  // follow the workgroup's own most recent loads of those columns.
+  const int nk = a.K / KC;
+  const bool res_in_x = a.Res != nullptr && (const void*)a.Res == a.X && a.ldres == a.ldx && col0 + BN <= a.K;
+  const int kst = res_in_x ? ((col0 + BN) / KC) % nk : 0;
+  auto kof = [&](int it) __attribute__((always_inline)) {
+    const int c = it + kst;
+    return (c >= nk ? c - nk : c) * KC;
+  };
+  gload(kof(0));
+  bload(kof(0));
+  for (int it = 0; it < nk; ++it) {
     lstore();
     // this chunk's B fragments.  FP32: the next chunk's are loaded while this
     // one is multiplied.  BF16 modes: the NS planes are used from the load
@@ -220,9 +231,9 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       }
     }
     __syncthreads();
-    if (k0 + KC < a.K) {
-      gload(k0 + KC);
-      if constexpr (F32) bload(k0 + KC);
+    if (it + 1 < nk) {
+      gload(kof(it + 1));
+      if constexpr (F32) bload(kof(it + 1));
     }
     if constexpr (!F32) {
 #pragma unroll
@@ -265,7 +276,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
           acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0][nb], acc[mb][nb], 0, 0, 0);
         }
       }
-      if (k0 + KC < a.K) bload(k0 + KC);
+      if (it + 1 < nk) bload(kof(it + 1));
     }
     __syncthreads();
   }
