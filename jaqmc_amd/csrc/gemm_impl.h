@@ -202,12 +202,14 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   }
 
   // When the residual is a slice of this GEMM's own input (X[row][n] for
-  // output column n), the K loop is rotated to end on the chunks that hold
-  // the tile's columns col0 .. col0+BN-1: the epilogue's residual reads then
-  // follow the workgroup's own most recent loads of those columns.
+  // output column n < ncols), the K loop starts after the residual columns
+  // and ends on them, so the epilogue's residual reads follow the row tile's
+  // most recent loads of those columns.  Every column block of a row tile
+  // walks K in the same order (they run back to back on one XCD and share
+  // the X panel's loads in its L2).
   const int nk = a.K / KC;
-  const bool res_in_x = a.Res != nullptr && (const void*)a.Res == a.X && a.ldres == a.ldx && col0 + BN <= a.K;
-  const int kst = res_in_x ? ((col0 + BN) / KC) % nk : 0;
+  const bool res_in_x = a.Res != nullptr && (const void*)a.Res == a.X && a.ldres == a.ldx && a.ncols <= a.K;
+  const int kst = res_in_x ? ((a.ncols + KC - 1) / KC) % nk : 0;
   auto kof = [&](int it) __attribute__((always_inline)) {
     const int c = it + kst;
     return (c >= nk ? c - nk : c) * KC;
