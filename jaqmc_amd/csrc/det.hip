@@ -452,13 +452,16 @@ __global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* 
 // no workgroup barriers.  The augmented [A | I] (16 x 32 FP64) is reduced by
 // Gauss-Jordan with partial pivoting in the wave's LDS slice; then, for each
 // coordinate c, B_c = A^-1 J_c is one 16x16 FP64 MFMA tile (four K steps,
-// A^-1 fragments in registers, J_c from HBM several coordinates ahead), with
+// A^-1 fragments in registers, J_c from HBM one coordinate pair ahead), with
 // tr(B_c) from the diagonal lanes and tr(B_c^2) from a transposed read of the
 // tile in the wave's LDS slice.
 // ---------------------------------------------------------------------------
 
 constexpr int SM_NP = 16;
-constexpr int SM_PF = 4;  // coordinates in flight in the small kernel (even)
+// coordinates in flight in the small kernel (even).  Two, with the kernel
+// bounded to 128 VGPRs (four waves per SIMD, 16 per CU), beat four at three
+// waves per SIMD: more waves hide the LDS transposes and wave barriers.
+constexpr int SM_PF = 2;
 struct SmallDetSmem {
   double M[SM_NP * 2 * SM_NP];  // augmented matrix, later Inv in columns 16..31
   double Tt[2 * SM_NP * 17];    // transpose tiles of B_c, B_c+1 (odd row stride)
@@ -469,7 +472,7 @@ struct SmallFwdSmem {           // forward mode: the matrix and the pivot exchan
 };
 
 template <bool LAPMODE>
-__global__ __launch_bounds__(256) void det_small_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+__global__ __launch_bounds__(256, 4) void det_small_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
                                                         const float* __restrict__ env, double* __restrict__ out) {
   constexpr bool lapmode = LAPMODE;
   using Smem = typename std::conditional<LAPMODE, SmallDetSmem, SmallFwdSmem>::type;
