@@ -159,24 +159,28 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
 // 16-byte access: the row-per-thread statistics passes read four features at
 // a time (a row's start moves by one 16-byte slot per thread: no bank
 // conflicts), and the output pass gives each thread four consecutive features
-// of every (256 / (D/4))-th row, written back as 16-byte stores.
-template <int D>
-__global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, const float* __restrict__ x,
+// of every (NTH / (D/4))-th row, written back as 16-byte stores.  NTH = 512
+// when the tile leaves room for one workgroup per CU only (two waves per
+// SIMD instead of one), else 256.  Rp <= 256 (one thread per row in the
+// statistics passes).
+template <int D, int NTH>
+__global__ __launch_bounds__(NTH) void layernorm_lap_lds_kernel(int Rp, int R, const float* __restrict__ x,
                                                                 float* __restrict__ y, const float* __restrict__ scale,
                                                                 const float* __restrict__ bias, float eps) {
   constexpr int SD = D + 4;
   constexpr int F4 = D / 4;       // float4 per row
-  constexpr int RG = 256 / F4;    // row groups of the output pass
-  static_assert(256 % F4 == 0, "D must divide 1024");
+  constexpr int RG = NTH / F4;    // row groups of the output pass
+  constexpr int NW = NTH / 64;    // waves
+  static_assert(NTH % F4 == 0 && NTH >= 256, "D must divide 4 * NTH");
   extern __shared__ __attribute__((aligned(16))) float tile[];  // Rp x SD
   __shared__ __attribute__((aligned(16))) float d0s[D];
-  __shared__ float mu[256], js[256], red[8];
+  __shared__ float mu[NTH], js[NTH], red[NW + 1];
   __shared__ __attribute__((aligned(16))) float crs[RG * D];  // partial cross sums per row group
   const long base = (long)blockIdx.x * Rp * D;
   const int tid = threadIdx.x;
   const float4* x4 = (const float4*)(x + base);
 #pragma unroll 8
-  for (int idx = tid; idx < Rp * F4; idx += 256) {
+  for (int idx = tid; idx < Rp * F4; idx += NTH) {
     const int r = idx / F4, q = idx - r * F4;
     *(float4*)(tile + r * SD + 4 * q) = x4[idx];
   }
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, c
     mu[tid] = m;
   }
   __syncthreads();
-  for (int f = tid; f < D; f += 256) d0s[f] = tile[f] - mu[0];
+  for (int f = tid; f < D; f += NTH) d0s[f] = tile[f] - mu[0];
   __syncthreads();
   float dot = 0.f, sq = 0.f;
   if (tid < Rp) {
@@ -225,10 +229,13 @@ __global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, c
   if (tid == ROW_LAP) p_lap += dot;
   p_lap = wave_sum(p_lap);
   if ((tid & 63) == 0) red[tid >> 6] = p_lap;
-  if (tid == 0) red[4] = dot;
+  if (tid == 0) red[NW] = dot;
   __syncthreads();
-  const float v = red[4] / D;
-  const float lapv = 2.f / D * (red[0] + red[1] + red[2] + red[3]);
+  const float v = red[NW] / D;
+  float lap_sum = red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) lap_sum += red[w];
+  const float lapv = 2.f / D * lap_sum;
   const float ve = v + eps;
   const float sv = rsqrtf(ve);
   const float s1 = -0.5f * sv / ve;
@@ -239,7 +246,10 @@ __global__ __launch_bounds__(256) void layernorm_lap_lds_kernel(int Rp, int R, c
   if ((tid & 63) == 0) red[tid >> 6] = p_jv2;
   js[tid] = s1 * jv;
   __syncthreads();
-  const float laps = s1 * lapv + s2 * (red[0] + red[1] + red[2] + red[3]);
+  float jv2_sum = red[0];
+#pragma unroll
+  for (int w = 1; w < NW; ++w) jv2_sum += red[w];
+  const float laps = s1 * lapv + s2 * jv2_sum;
   float* yu = y + base;
   const int f4 = tid % F4, ro = tid / F4, f = 4 * f4;
   const float4 dd = *(const float4*)&d0s[f];
@@ -392,20 +402,39 @@ int launch_psi_features(const FermiDims& d, const float* elec, const float* atom
   return 0;
 }
 
+// static LDS of layernorm_lap_lds_kernel<D, NTH> in bytes (d0s, mu, js, red,
+// crs), with room for alignment
+template <int D, int NTH>
+constexpr size_t ln_lds_fixed() {
+  return (size_t)(D + 2 * NTH + NTH / 64 + 1 + (NTH / (D / 4)) * D) * sizeof(float) + 64;
+}
+
+template <int D, int NTH>
+bool launch_ln_lds_n(int units, int Rp, int R, const float* x, float* y, const float* scale, const float* bias,
+                     float eps, hipStream_t s) {
+  const size_t smem = (size_t)Rp * (D + 4) * sizeof(float);
+  const size_t fixed = ln_lds_fixed<D, NTH>();
+  if (Rp > 256 || smem + fixed > 160 * 1024) return false;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)layernorm_lap_lds_kernel<D, NTH>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)(160 * 1024 - fixed));
+    attr = true;
+  }
+  hipLaunchKernelGGL((layernorm_lap_lds_kernel<D, NTH>), dim3(units), dim3(NTH), smem, s, Rp, R, x, y, scale, bias,
+                     eps);
+  return true;
+}
+
+// 512 threads when the tile allows one workgroup per CU only, else 256
 template <int D>
 bool launch_ln_lds(int units, int Rp, int R, const float* x, float* y, const float* scale, const float* bias,
                    float eps, hipStream_t s) {
   const size_t smem = (size_t)Rp * (D + 4) * sizeof(float);
-  const size_t fixed = (D + 2 * 256 + 8 + 1024) * sizeof(float);
-  if (smem + fixed > 160 * 1024) return false;
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)layernorm_lap_lds_kernel<D>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)(160 * 1024 - fixed));
-    attr = true;
-  }
-  hipLaunchKernelGGL(layernorm_lap_lds_kernel<D>, dim3(units), dim3(256), smem, s, Rp, R, x, y, scale, bias, eps);
-  return true;
+  if (smem + ln_lds_fixed<D, 256>() > 80 * 1024 &&
+      launch_ln_lds_n<D, 512>(units, Rp, R, x, y, scale, bias, eps, s))
+    return true;
+  return launch_ln_lds_n<D, 256>(units, Rp, R, x, y, scale, bias, eps, s);
 }
 
 int launch_layernorm_lap(int units, int Rp, int R, int D, const float* x, float* y, const float* scale,
