@@ -256,9 +256,11 @@ __global__ void pair_layer_generic_kernel(long npairs, const float* __restrict__
 //   d/dx_j   <- (d/dx_a rows of pair (j, i)) / n_s,   j in s
 // One workgroup per (walker, electron) stages the column H2[w][:, i] in LDS
 // and writes the unit's whole block X[unit][0:Rp][col0:ldx) row by row;
-// columns past nch*f2 are zero-filled (GEMM K padding).
-template <typename T>
-__global__ __launch_bounds__(256) void pair_means_kernel(FermiDims d, int RP2, const float* __restrict__ H2, int f2,
+// columns past nch*f2 are zero-filled (GEMM K padding).  NTH threads: 1024
+// when the staged column is large enough to leave room for few workgroups
+// per CU (large N), else 256.
+template <typename T, int NTH>
+__global__ __launch_bounds__(NTH) void pair_means_kernel(FermiDims d, int RP2, const float* __restrict__ H2, int f2,
                                                          T* __restrict__ X, long ldx, int col0) {
   extern __shared__ float col[];  // [j][RP2][f2]
   float* part = col + d.N * RP2 * f2;  // [s][5][f2]: sums of value, lap, d/dx_i (3)
@@ -270,20 +272,20 @@ __global__ __launch_bounds__(256) void pair_means_kernel(FermiDims d, int RP2, c
     // 16-byte loads, eight in flight per thread
     const int q4 = RP2 * f2 / 4;
 #pragma unroll 8
-    for (int idx = tid; idx < d.N * q4; idx += 256) {
+    for (int idx = tid; idx < d.N * q4; idx += NTH) {
       const int j = idx / q4, q = idx - j * q4;
       *(float4*)&col[4 * idx] = *(const float4*)(base + (long)j * d.N * pstride + 4 * q);
     }
   } else {
 #pragma unroll 8
-    for (int idx = tid; idx < d.N * RP2 * f2; idx += 256) {
+    for (int idx = tid; idx < d.N * RP2 * f2; idx += NTH) {
       const int j = idx / (RP2 * f2), q = idx % (RP2 * f2);
       col[idx] = base[(long)j * d.N * pstride + q];
     }
   }
   __syncthreads();
   const int nrow_sum = RP2 > 1 ? 5 : 1;
-  for (int idx = tid; idx < d.nch * nrow_sum * f2; idx += 256) {
+  for (int idx = tid; idx < d.nch * nrow_sum * f2; idx += NTH) {
     const int s = idx / (nrow_sum * f2), q = (idx / f2) % nrow_sum, f = idx % f2;
     const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
     const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
@@ -296,14 +298,14 @@ __global__ __launch_bounds__(256) void pair_means_kernel(FermiDims d, int RP2, c
   const int u = unit_of(w, i, d.Bc, d.n_up, d.n_dn);
   const int ncol = (int)(ldx - col0);
   T* xu = X + (long)u * d.Rp * ldx + col0;
-  // lanes walk columns (per-column constants hoisted), the four waves walk rows
+  // lanes walk columns (per-column constants hoisted), the waves walk rows
   for (int cc = tid & 63; cc < ncol; cc += 64) {
     const bool live = cc < d.nch * f2;
     const int s = live ? cc / f2 : 0, f = live ? cc - s * f2 : 0;
     const int j0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
     const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
     const float inv = 1.f / ns;
-    for (int r = tid >> 6; r < d.Rp; r += 4) {
+    for (int r = tid >> 6; r < d.Rp; r += NTH / 64) {
       float out = 0.f;
       if (live) {
         if (r == ROW_VAL || r == ROW_LAP) {
@@ -543,10 +545,17 @@ int launch_pair_means(const FermiDims& d, const float* H2, int f2, void* X, long
   const size_t smem = ((size_t)d.N * RP2 * f2 + 2 * 5 * f2) * 4;
   if (smem > 64 * 1024) return -1;
   const dim3 grid((unsigned)((long)d.Bc * d.N));
-  if (x_bf16)
-    hipLaunchKernelGGL(pair_means_kernel<bf16>, grid, 256, smem, s, d, RP2, H2, f2, (bf16*)X, ldx, col0);
-  else
-    hipLaunchKernelGGL(pair_means_kernel<float>, grid, 256, smem, s, d, RP2, H2, f2, (float*)X, ldx, col0);
+  if (smem > 40 * 1024) {
+    if (x_bf16)
+      hipLaunchKernelGGL((pair_means_kernel<bf16, 1024>), grid, 1024, smem, s, d, RP2, H2, f2, (bf16*)X, ldx, col0);
+    else
+      hipLaunchKernelGGL((pair_means_kernel<float, 1024>), grid, 1024, smem, s, d, RP2, H2, f2, (float*)X, ldx, col0);
+  } else {
+    if (x_bf16)
+      hipLaunchKernelGGL((pair_means_kernel<bf16, 256>), grid, 256, smem, s, d, RP2, H2, f2, (bf16*)X, ldx, col0);
+    else
+      hipLaunchKernelGGL((pair_means_kernel<float, 256>), grid, 256, smem, s, d, RP2, H2, f2, (float*)X, ldx, col0);
+  }
   return 0;
 }
 
