@@ -324,20 +324,23 @@ __global__ __launch_bounds__(NTH) void pair_means_kernel(FermiDims d, int RP2, c
   }
 }
 
-// Spin-channel means of the one-electron stream, one workgroup per
-// (walker, channel): g[w][r][s*f1 + f] = mean_{i in s} X[unit(w, i)][r][f]
+// Spin-channel means of the one-electron stream, rsplit workgroups per
+// (walker, channel), each over a slice of the rows: g[w][r][s*f1 + f] = mean_{i in s} X[unit(w, i)][r][f]
 // with float4 loads and stores (f1 % 4 == 0); columns >= nch*f1 of g are
-// zero-filled by the s = nch-1 workgroup.
-__global__ __launch_bounds__(256) void spin_means4_kernel(FermiDims d, const float* __restrict__ X, long ldx, int f1,
-                                                          float* __restrict__ g, long ldg) {
-  const int w = blockIdx.x / d.nch, s = blockIdx.x % d.nch;
+// zero-filled by the s = nch-1 workgroups, each for its rows.
+__global__ __launch_bounds__(256) void spin_means4_kernel(FermiDims d, int rsplit, const float* __restrict__ X,
+                                                          long ldx, int f1, float* __restrict__ g, long ldg) {
+  const int ws = blockIdx.x / rsplit, rs = blockIdx.x % rsplit;
+  const int w = ws / d.nch, s = ws % d.nch;
+  const int rows = (d.Rp + rsplit - 1) / rsplit, r0 = rs * rows;
+  const int nr = d.Rp - r0 < rows ? d.Rp - r0 : rows;  // <= 0: nothing to do
   const int i0 = d.nch == 1 ? 0 : (s == 0 ? 0 : d.n_up);
   const int ns = d.nch == 1 ? d.N : (s == 0 ? d.n_up : d.n_dn);
   const int u0 = unit_of(w, i0, d.Bc, d.n_up, d.n_dn);  // units of (w, s) are consecutive
   const float inv = 1.f / ns;
   const int q4 = f1 / 4;
-  for (int idx = threadIdx.x; idx < d.Rp * q4; idx += 256) {
-    const int r = idx / q4, f = (idx % q4) * 4;
+  for (int idx = threadIdx.x; idx < nr * q4; idx += 256) {
+    const int r = r0 + idx / q4, f = (idx % q4) * 4;
     float4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 4
     for (int k = 0; k < ns; ++k) {
@@ -355,8 +358,8 @@ __global__ __launch_bounds__(256) void spin_means4_kernel(FermiDims d, const flo
   }
   if (s == d.nch - 1) {
     const int pad = (int)(ldg - d.nch * f1);
-    for (int idx = threadIdx.x; idx < d.Rp * pad; idx += 256) {
-      const int r = idx / pad, c = idx % pad;
+    for (int idx = threadIdx.x; idx < nr * pad; idx += 256) {
+      const int r = r0 + idx / pad, c = idx % pad;
       g[((long)w * d.Rp + r) * ldg + d.nch * f1 + c] = 0.f;
     }
   }
@@ -562,7 +565,13 @@ int launch_pair_means(const FermiDims& d, const float* H2, int f2, void* X, long
 int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void* g, long ldg, bool bf16io,
                       hipStream_t s) {
   if (!bf16io && f1 % 4 == 0 && ldx % 4 == 0 && ldg % 4 == 0) {
-    hipLaunchKernelGGL(spin_means4_kernel, dim3((unsigned)((long)d.Bc * d.nch)), 256, 0, s, d, (const float*)X, ldx,
+    // rows of a (walker, channel) split over several workgroups when there
+    // are few walkers, so the launch has a few thousand workgroups
+    const long wg = (long)d.Bc * d.nch;
+    int rsplit = (int)((4096 + wg - 1) / wg);
+    const int rmax = d.Rp >= 16 ? d.Rp / 8 : 1;
+    if (rsplit > rmax) rsplit = rmax;
+    hipLaunchKernelGGL(spin_means4_kernel, dim3((unsigned)(wg * rsplit)), 256, 0, s, d, rsplit, (const float*)X, ldx,
                        f1, (float*)g, ldg);
     return 0;
   }
