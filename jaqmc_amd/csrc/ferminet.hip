@@ -404,7 +404,7 @@ __global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float*
                                                        const float* __restrict__ sg0, const float* __restrict__ pi1,
                                                        const float* __restrict__ sg1, int kind,
                                                        float* __restrict__ env) {
-  extern __shared__ float geo[];  // [M][4]: dx, dy, dz, r
+  extern __shared__ float geo[];  // [M][4]: dx, dy, dz, r; then [M]: 1 / r
   const int wi = blockIdx.x, i = wi % d.N;
   const int KN = d.K * d.N;
   float* out = env + (long)wi * KN * 5;
@@ -421,7 +421,9 @@ __global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float*
     geo[4 * I] = dx;
     geo[4 * I + 1] = dy;
     geo[4 * I + 2] = dz;
-    geo[4 * I + 3] = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+    geo[4 * I + 3] = r;
+    geo[4 * d.M + I] = 1.f / r;
   }
   __syncthreads();
   const bool up = d.nch == 1 || i < d.n_up;
@@ -431,11 +433,12 @@ __global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float*
     float v = 0.f, l = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     for (int I = 0; I < d.M; ++I) {
       const float4 g = *(const float4*)&geo[4 * I];
+      const float ir = geo[4 * d.M + I];  // one division per atom and workgroup
       const float a = kind == 1 ? fabsf(sg[(long)I * KN + ko]) : sg[(long)I * KN + ko];
       const float f = pi[(long)I * KN + ko] * expf(-a * g.w);
       v += f;
-      l += f * (a * a - 2.f * a / g.w);
-      const float gs = -a * f / g.w;
+      l += f * a * (a - 2.f * ir);
+      const float gs = -a * f * ir;
       gx += gs * g.x;
       gy += gs * g.y;
       gz += gs * g.z;
@@ -588,7 +591,7 @@ int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void*
 int launch_envelope(const FermiDims& d, const float* elec, const float* atoms, const float* const* pi,
                     const float* const* sigma, int env_kind, float* env, hipStream_t s) {
   if ((long)d.Bc * d.N <= 0) return 0;
-  const size_t smem = (size_t)4 * d.M * sizeof(float);
+  const size_t smem = (size_t)5 * d.M * sizeof(float);
   if (smem > 64 * 1024) return -1;
   hipLaunchKernelGGL(envelope_kernel, dim3((unsigned)(d.Bc * d.N)), 256, smem, s, d, elec, atoms, pi[0], sigma[0],
                      d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
