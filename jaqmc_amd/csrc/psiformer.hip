@@ -33,7 +33,14 @@ __global__ void psi_features_kernel(FermiDims d, const float* __restrict__ elec,
     const int u = ur / d.Rp;
     const int w = u / d.N, i = u % d.N;
     float out = 0.f;
+    // rows other than the value, the Laplacian and electron i's own three
+    // Jacobian rows are zero: whole waves of them skip the geometry
+    const bool live = r == ROW_VAL || r == ROW_LAP || (r - ROW_J0 < 3 * d.N && (r - ROW_J0) / 3 == i);
     if (f < 4 * d.M) {
+      if (!live) {
+        X[ur * kin + f] = 0.f;
+        continue;
+      }
       const int I = f >> 2, e = f & 3;
       const float* xi = elec + ((long)w * d.N + i) * 3;
       const float dv[3] = {xi[0] - atoms[3 * I], xi[1] - atoms[3 * I + 1], xi[2] - atoms[3 * I + 2]};
