@@ -8,9 +8,15 @@ plus the FP64 mean / variance of E_L over the batch -- the reference's
 EvaluationWorkStage.compute_step with MCMCSampler(steps=1).  A timed *step*
 is one walker-step of the whole batch; throughput = B * steps / wall.
 
-Default workload: N2-like diatomic, 14 electrons (7/7), FermiNet 4 layers
-256/32, 16 determinants, 4096 walkers, proposal width 0.02, seeded random
-parameters (synthetic data; no checkpoint exists for this system).
+Default workload: the 42-electron planar ring (6 C on a radius-2.64 hexagon,
+6 H at 4.68 bohr), Psiformer with 4 layers, 4 heads, model width 256, MLP
+[256], 16 determinants, Jastrow, 2048 walkers, proposal width 0.02; seeded
+random parameters (synthetic data; no checkpoint exists for this system).
+Both the plain FP32 path and the mixed path are timed; `value` is the mixed
+path.  Beside them: the configuration's protocol time (S Metropolis steps then
+one local energy), the paths' paired errors against FP64 on oracle-equilibrated
+walkers with their accept/reject agreement (committed fixtures), the roofline
+of the dominant kernel, and the CPU baseline.
 
 Prints ONE JSON line on stdout; diagnostics go to stderr.
 """
@@ -20,24 +26,30 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
 import numpy as np
 
+ROOT = os.path.dirname(os.path.abspath(__file__))
 
-def measured_traffic(workload: str):
+
+def measured_traffic(workload: str, mixed: str):
     """HBM bytes per dispatch of the mixed-precision layer GEMM of this workload,
     from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated;
     see jaqmc_amd/profiling.py), or None when no summary is committed."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"gemm_layer_lap_{workload}_traffic.txt")
-    if not os.path.exists(path):
-        return None, None
-    with open(path) as f:
-        for line in f:
+    for name in (f"gemm_layer_lap_{workload}_{mixed}_traffic.txt", f"gemm_layer_lap_{workload}_traffic.txt"):
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            txt = f.read()
+        if mixed not in txt and name.endswith(f"{workload}_traffic.txt") and mixed != "bf16x6":
+            continue  # the generic summary is the bf16x6 kernel's
+        for line in txt.splitlines():
             if line.startswith("traffic_bytes_per_dispatch:"):
-                return float(line.split(":", 1)[1]), "profiles/" + os.path.basename(path)
+                return float(line.split(":", 1)[1]), "profiles/" + name
     return None, None
 
 
@@ -45,7 +57,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(spec, system, B, width, params, x0, precision, steps, warmup):
+def run_path(spec, system, B, width, params, x0, precision, steps, warmup, S_steps):
     import torch
 
     from jaqmc_amd import _lib
@@ -81,16 +93,26 @@ def run_path(spec, system, B, width, params, x0, precision, steps, warmup):
     mv = mv.cpu().numpy()
     res = dict(wall=wall, ms_per_step=1e3 * wall / steps, value=B * steps / wall, prof=prof,
                mean=float(mv[0]), var=float(mv[1]), pmove=float(acc.sum().item()) / (B * (steps + warmup)))
-    res["x_final"] = x.cpu().numpy()
+    # the configuration's protocol: S Metropolis steps, then one local energy
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dm.mh_steps(x, logp, S_steps, width, seed=99, ctr=0, n_accept=acc)
+    torch.cuda.synchronize()
+    t_mh = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = dm.local_energy(x)
+    reduce_mean_var(out["e_loc"])
+    torch.cuda.synchronize()
+    t_el = time.perf_counter() - t0
+    res["protocol"] = {"mh_steps": S_steps, "mh_s": t_mh, "local_energy_s": t_el,
+                       "walker_steps_per_s": B * S_steps / (t_mh + t_el)}
     dm.close()
     return res
 
 
 def paired_stats(a, b):
     """Per-walker differences a - b (Hartree in, mHa out): the mean with its
-    standard error, a 1%-trimmed mean, and the spread of |a - b|.  E_L has heavy
-    tails (walkers near nodes or coalescences), so the mean alone is dominated by
-    a few walkers."""
+    standard error, a 1%-trimmed mean, and the spread of |a - b|."""
     d = 1e3 * (np.asarray(a, np.float64) - np.asarray(b, np.float64))
     n = d.size
     keep = np.argsort(np.abs(d))[: max(1, int(np.floor(0.99 * n)))]
@@ -100,51 +122,97 @@ def paired_stats(a, b):
             "trimmed1pct_mean_mHa": float(d[keep].mean()),
             "median_abs_mHa": float(np.median(np.abs(d))), "p99_abs_mHa": float(np.quantile(np.abs(d), 0.99)),
             "max_abs_mHa": float(np.abs(d).max()), "median_rel": float(np.median(rel)),
-            "p99_rel": float(np.quantile(rel, 0.99))}
+            "p99_rel": float(np.quantile(rel, 0.99)), "max_rel": float(rel.max()),
+            "frac_within_1e-5_rel": float(np.mean(rel <= 1e-5))}
 
 
-def energies_on(spec, system, params, x, precision):
+def fixture_accuracy(spec, system, params, workload, precisions, width):
+    """Paired E_L errors against FP64 and accept/reject agreement on the
+    committed oracle-equilibrated walkers (tests/golden/equilibrated_<w>.npz,
+    written by tests/golden/make_equilibrated.py in the build container)."""
     import torch
 
     from jaqmc_amd.device import DeviceModel
 
-    dm = DeviceModel(spec, system.atoms, system.charges, params, precision=precision)
-    out = dm.local_energy(torch.from_numpy(x).cuda())
-    e = out["e_loc"].cpu().numpy()
-    dm.close()
-    return e
+    path = os.path.join(ROOT, "tests", "golden", f"equilibrated_{workload}.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as f:
+        fx = {k: f[k] for k in f.files}
+    x = torch.from_numpy(np.ascontiguousarray(fx["x"])).cuda()
+    n = x.shape[0]
+    info = {"walkers": int(n), "source": "tests/golden/" + os.path.basename(path),
+            "equilibration": f"{int(fx['steps'])} FP64 oracle Metropolis steps",
+            "fp64_mean_Ha": float(fx["e_loc"].mean())}
+    for prec in precisions:
+        dm = DeviceModel(spec, system.atoms, system.charges, params, precision=prec)
+        e = dm.local_energy(x)["e_loc"].cpu().numpy()
+        st = paired_stats(e, fx["e_loc"])
+        # one Metropolis move with the fixture's normals and uniforms
+        xm = x.clone()
+        logp = 2.0 * dm.logpsi(xm)[0]
+        acc = dm.mh_steps(xm, logp, 1, width, normals=torch.from_numpy(fx["mh_normals"]).cuda(),
+                          uniforms=torch.from_numpy(fx["mh_uniforms"]).cuda())
+        a = acc.cpu().numpy().astype(bool)
+        amb = np.abs(fx["mh_ratio"] - np.log(fx["mh_uniforms"].astype(np.float64))) < 1e-6
+        st["accept_agreement"] = {"decisions": int(n), "excluded_within_1e-6": int(amb.sum()),
+                                  "agree": int(np.sum((a == fx["mh_accept"]) & ~amb)),
+                                  "disagree": int(np.sum((a != fx["mh_accept"]) & ~amb))}
+        info[f"{prec}_vs_fp64"] = st
+        dm.close()
+    return info
 
 
 def cpu_baseline(spec, system, params, x, width):
-    """Oracle (numpy FP32 restatement) walker-steps/s on a bounded sample."""
+    """Oracle (numpy FP32 restatement) walker-steps/s on a bounded sample,
+    with the BLAS thread count set explicitly."""
+    from threadpoolctl import threadpool_limits
+
     from oracle import hamiltonian as ham
     from oracle import wavefunction as owf
 
+    try:
+        import psutil
+
+        phys = psutil.cpu_count(logical=False) or os.cpu_count() or 1
+    except Exception:
+        phys = os.cpu_count() or 1
+    threads = int(min(phys, 16))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     m = spec.as_oracle()
     rng = np.random.default_rng(7)
-    t0 = time.perf_counter()
-    _, lp = owf.logpsi(m, params, x, system.atoms, with_laplacian=False)
-    t_init = time.perf_counter() - t0
-    normals = rng.standard_normal(x.shape).astype(np.float32)
-    u = rng.uniform(size=x.shape[0]).astype(np.float32)
-    t0 = time.perf_counter()
-    xn, _, _, _, _ = ham.mh_step(m, params, x, 2 * lp.v, system.atoms, width, normals, u)
-    ham.local_energy(m, params, xn, system.atoms, system.charges)
-    t = time.perf_counter() - t0
-    del t_init
-    return x.shape[0] / t, t
+    with threadpool_limits(limits=threads):
+        _, lp = owf.logpsi(m, params, x, system.atoms, with_laplacian=False)
+        normals = rng.standard_normal(x.shape).astype(np.float32)
+        u = rng.uniform(size=x.shape[0]).astype(np.float32)
+        t0 = time.perf_counter()
+        xn, _, _, _, _ = ham.mh_step(m, params, x, 2 * lp.v, system.atoms, width, normals, u)
+        ham.local_energy(m, params, xn, system.atoms, system.charges)
+        t = time.perf_counter() - t0
+    return x.shape[0] / t, t, threads, phys, model
+
+
+CPU_SAMPLE = {"li": 1024, "n2": 512, "ring42": 6, "h60": 2}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="n2", choices=["li", "n2", "ring42", "h60"])
-    ap.add_argument("--mixed", default="bf16x6", choices=["bf16x6", "bf16x3", "bf16"])
-    ap.add_argument("--cpu-sample", type=int, default=256, help="walkers in the CPU-baseline sample")
-    ap.add_argument("--err-sample", type=int, default=64, help="walkers checked against the FP64 oracle")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ring42", choices=["li", "n2", "ring42", "h60"])
+    ap.add_argument("--mixed", default="fp16x3", choices=["fp16x3", "bf16x6", "bf16x3", "bf16"])
+    ap.add_argument("--cpu-sample", type=int, default=0, help="walkers in the CPU-baseline sample (0: per workload)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=12, help="jobs in the serving sample (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate of the serving sample")
     ap.add_argument("--serving-steps", type=int, default=20, help="evaluation iterations per serving job")
@@ -158,7 +226,7 @@ def main():
               "a single one-GPU figure is what this build reports.", file=sys.stderr)
         sys.exit(2)
 
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, ROOT)
     import torch
 
     from jaqmc_amd import params as P
@@ -185,35 +253,14 @@ def main():
 
     res = {}
     for prec in ("fp32", args.mixed):
-        res[prec] = run_path(spec, system, B, width, params, x0, prec, args.steps, args.warmup)
-        log(prec, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res[prec].items()
-                   if k not in ("prof", "x_final")})
+        res[prec] = run_path(spec, system, B, width, params, x0, prec, args.steps, args.warmup, S_steps)
+        log(prec, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res[prec].items() if k != "prof"})
         log("  prof (ms total, launches):", res[prec]["prof"])
 
-    # --- accuracy: FP32 and mixed E_L against the FP64 oracle on a sample, and
-    # mixed against FP32 on the full batch; on the initial walkers and on the
-    # walkers after the timed Metropolis steps ---
-    from oracle import hamiltonian as ham
-
-    ns = min(args.err_sample, B)
-    p64 = P.cast_params(params, np.float64)
-    acc_info = {"note": "paired per-walker differences; E_L is heavy-tailed, see stderr and trimmed mean"}
-    for tag, xs in (("initial_walkers", x0), (f"after_{args.warmup + args.steps}_mh_steps",
-                                               res[args.mixed]["x_final"])):
-        e32 = energies_on(spec, system, params, xs, "fp32")
-        emx = energies_on(spec, system, params, xs, args.mixed)
-        t0 = time.perf_counter()
-        ref = ham.local_energy(spec.as_oracle(), p64, xs[:ns].astype(np.float64), system.atoms.astype(np.float64),
-                               system.charges.astype(np.float64))["e_loc"]
-        log(f"FP64 oracle on {ns} walkers ({tag}): {time.perf_counter() - t0:.1f}s")
-        acc_info[tag] = {
-            "fp32_vs_fp64_sample": paired_stats(e32[:ns], ref),
-            "mixed_vs_fp64_sample": paired_stats(emx[:ns], ref),
-            "mixed_vs_fp32_full_batch": paired_stats(emx, e32),
-            "full_batch_mean_Ha": {"fp32": float(e32.mean()), "mixed": float(emx.mean()),
-                                   "mc_stderr": float(e32.std(ddof=1) / np.sqrt(B))},
-        }
-    log("accuracy:", json.dumps(acc_info))
+    acc_info = None
+    if not args.no_accuracy:
+        acc_info = fixture_accuracy(spec, system, params, args.workload, ("fp32", args.mixed), width)
+        log("accuracy:", json.dumps(acc_info))
 
     # --- roofline of the dominant kernel class (layer GEMMs in Laplacian mode) ---
     # achieved = algorithmic FLOPs of all layer-GEMM launches in the timed region
@@ -222,15 +269,10 @@ def main():
     mixed = res[args.mixed]
     tot_ms, n = mixed["prof"]["gemm_layer_lap"]
     achieved = fl["gemm_layer_lap"] * B * args.steps / (tot_ms * 1e-3) if n else 0.0
-    peak = RF.PEAKS["bf16"]
-    # Context for the fraction: each FP32-accurate product costs `mult` BF16
-    # MFMA products, so issued MFMA work is mult x the algorithmic FLOPs; the
-    # minimum HBM bytes (operands once, outputs once) give the memory side.
-    # Algorithmic intensity is ~60 FLOP/B, above the emulated ridge
-    # (2.5 PFLOP/s / mult / 8 TB/s = 52 FLOP/B for bf16x6), so MFMA bounds it.
+    peak = RF.PEAKS[RF.MODE_PEAK[args.mixed]]
     mult = RF.MFMA_PER_PRODUCT[args.mixed]
     alg_bytes = RF.layer_bytes(spec, system.atoms.shape[0]) * B * args.steps
-    traffic, traffic_src = measured_traffic(args.workload)
+    traffic, traffic_src = measured_traffic(args.workload, args.mixed)
     avg_s = tot_ms * 1e-3 / max(n, 1)
     roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": traffic,
@@ -246,11 +288,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        nc = min(args.cpu_sample, B)
-        v, t = cpu_baseline(spec, system, params, x0[:nc], width)
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": v, "unit": "walker-steps/s", "cores": cores, "kind": "port",
-               "sample": f"{nc} walkers x 1 walker-step (FP32 numpy oracle, MH move + E_L), {t:.1f}s"}
+        nc = min(args.cpu_sample or CPU_SAMPLE[args.workload], B)
+        v, t, threads, phys, model = cpu_baseline(spec, system, params, x0[:nc], width)
+        cpu = {"value": v, "unit": "walker-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{nc} walkers x 1 walker-step (FP32 numpy oracle: MH move + dense-Jacobian E_L), "
+                         f"{t:.1f}s, BLAS threads {threads}",
+               "cpu_model": model, "physical_cores": phys, "machine": platform.machine()}
 
     arch = (f"FermiNet {len(spec.hidden_single)}x{spec.hidden_single[0]}/{spec.hidden_double[0]}"
             if spec.kind == "ferminet" else
@@ -273,8 +316,10 @@ def main():
                              "model": f"FermiNet 4x256/32 K=16, {args.mixed}"})
         log("serving:", json.dumps(serving_info))
 
+    names = {"li": "C1 Li atom", "n2": "C2 N2-like diatomic", "ring42": "C3 42-electron planar ring",
+             "h60": "C4 linear H60 chain"}
     line = {
-        "metric": f"walker-steps/s incl. E_L ({system.name} {arch}, BF16-mixed; FP32 beside it)",
+        "metric": f"walker-steps/s incl. E_L ({names[args.workload]}, {arch}; {args.mixed}-mixed, FP32 beside it)",
         "value": mixed["value"],
         "unit": "walker-steps/s",
         "n_gpus": 1,
@@ -286,11 +331,13 @@ def main():
         "vs_baseline": None,
         "dtype": f"{args.mixed}+fp64",
         "data": "synthetic (seeded random-init params, Gaussian walkers)",
-        "config": {"workload": f"{system.name} {arch} K={spec.ndets}", "walkers": B,
-                   "electrons": spec.n_elec, "mh_width": width},
+        "config": {"workload": f"{names[args.workload]}: {system.name} {arch} K={spec.ndets}", "walkers": B,
+                   "electrons": spec.n_elec, "mh_width": width, "protocol_mh_steps": S_steps},
         "fp32_value": res["fp32"]["value"],
         "fp32_ms_per_step": res["fp32"]["ms_per_step"],
         "mixed_over_fp32": mixed["value"] / res["fp32"]["value"],
+        "protocol": {"mixed": mixed["protocol"], "fp32": res["fp32"]["protocol"],
+                     "note": "the configuration's literal sequence: S Metropolis steps, then one local energy"},
         "accuracy": acc_info,
         "roofline": roof,
         "cpu_baseline": cpu,

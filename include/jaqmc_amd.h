@@ -37,6 +37,30 @@ extern "C" {
 #define JQ_ENV_ISOTROPIC 0
 #define JQ_ENV_ABS_ISOTROPIC 1
 #define JQ_ENV_NULL 2
+#define JQ_ENV_DIAGONAL 3      /* EnvelopeType.diagonal: rejected (JQ_EINVAL)        */
+
+/* PsiformerLayer.layer_norm_mode (wavefunction/backbone/psiformer.py:13-26,70-98) */
+#define JQ_LN_PRE 0
+#define JQ_LN_POST 1
+#define JQ_LN_NULL 2
+
+/* jq_model_desc.options: module options of the reference that change the
+ * computation.  A pack always carries every bias slot of the default module;
+ * a module built without a bias (with_bias / input_bias false) passes zeros
+ * there and sets the flag, and create rejects a flag whose slots are not all
+ * zero.  Unsupported options return JQ_EINVAL before any allocation. */
+#define JQ_OPT_NO_ATTN_BIAS (1 << 0)   /* Psiformer with_bias=False (psiformer.py:57,80)            */
+#define JQ_OPT_NO_INPUT_BIAS (1 << 1)  /* Psiformer input_bias=False (backbone/psiformer.py:175)    */
+#define JQ_OPT_ORBITAL_BIAS (1 << 2)   /* bias_orbitals=True: pack holds one (K*N) bias per orbital
+                                          kernel (output/orbital.py:60-78)                         */
+#define JQ_OPT_NO_RESCALE (1 << 3)     /* Psiformer rescale=False: [r, d] features (input/atomic.py) */
+#define JQ_OPT_SHARED_ORBITALS (1 << 4)/* orbitals_spin_split=False: one orbital kernel and one
+                                          envelope for both channels (orbital.py:73-76)            */
+#define JQ_OPT_USE_LAST_LAYER (1 << 5) /* FermiNet use_last_layer=True: rejected (JQ_EINVAL)      */
+/* Library option (no reference counterpart): allocate the parameter-gradient
+ * workspace at create, for jq_logpsi_vjp (FermiNet, hidden widths multiples
+ * of 32). */
+#define JQ_OPT_WITH_GRADIENTS (1 << 16)
 
 /* Arithmetic of the layer / tangent GEMMs.  Determinants (LU, inverse,
  * Laplacian traces) and walker reductions are FP64 in every mode. */
@@ -45,8 +69,12 @@ extern "C" {
 #define JQ_PREC_BF16X3 2 /* two-term split BF16 (hh + hl + lh), FP32 accumulate    */
 #define JQ_PREC_BF16X6 3 /* three-term split BF16 (6 products), FP32 accumulate:
                             FP32-level accuracy on BF16 matrix cores              */
-/* Activations are FP32 in HBM in every mode; operands are split into BF16
- * terms when staged into LDS. */
+#define JQ_PREC_FP16X3 4 /* two-term split FP16 (hh + hl + lh), FP32 accumulate:
+                            2^-22 operand error on 16-bit matrix cores; each
+                            unit's Laplacian row is scaled by a power of two
+                            before the split so it stays in FP16 range        */
+/* Activations are FP32 in HBM in every mode; operands are split into 16-bit
+ * terms when staged into LDS, weights are uploaded pre-split. */
 
 #define JQ_MAX_LAYERS 8
 
@@ -69,6 +97,8 @@ typedef struct jq_model_desc {
   int chunk_walkers;     /* walkers per workspace chunk (0: library default)     */
   const float* atoms;    /* HOST (n_atoms, 3); copied at create                  */
   const float* charges;  /* HOST (n_atoms,);  copied at create                   */
+  int layer_norm_mode;   /* JQ_LN_* (Psiformer; 0 = the reference default)       */
+  int options;           /* JQ_OPT_* bits (0 = the reference defaults)           */
 } jq_model_desc;
 
 typedef struct jq_model jq_model;
@@ -80,6 +110,8 @@ typedef struct jq_model jq_model;
  * app/molecule/wavefunction/psiformer.py:93-134). */
 int jq_model_create(const jq_model_desc* desc, const float* packed_params, size_t n_floats,
                     jq_model** out);
+/* Destroy: waits only for the streams this model was used on (events recorded
+ * at each call) and releases its memory stream-ordered; no device-wide sync. */
 int jq_model_destroy(jq_model* m);
 /* Bytes of device workspace the model holds (weights + chunk buffers). */
 size_t jq_model_workspace_bytes(const jq_model* m);
@@ -112,6 +144,46 @@ int jq_mh_steps(jq_model* m, float* elec, float* logp, int B, int n_steps, float
 int jq_local_energy(jq_model* m, const float* elec, int B, double* e_kin, double* e_pot,
                     double* e_loc, float* logpsi, float* grad, void* stream);
 
+/* Orbital matrices A[w, k, i, o] = Phi[k, i, o] * env[k, i, o] (FP32), the
+ * (B, K, N, N) array of MoleculeWavefunction.orbitals / get_orbitals
+ * (app/molecule/wavefunction/ferminet.py:80-96,128-138;
+ * app/molecule/wavefunction/psiformer.py:160-166,202-212).  Row = electron,
+ * column = orbital. */
+int jq_orbitals(jq_model* m, const float* elec, int B, float* orbitals, void* stream);
+
+/* Parameter gradient of log|psi| contracted with per-walker weights:
+ *   grad = sum_w cot[w] d log|psi(x_w)| / d theta      (FP32, pack order)
+ * the per-walker value_and_grad of LossAndGrad.evaluate_single_walker
+ * (estimator/loss_grad.py:70-103) reduced with the walker weights of
+ * finalize_stats (:115-128; jq_loss_clip's weights give the VMC energy
+ * gradient).  cot (B,) float64 on the device; a walker with cot 0 contributes
+ * exactly 0 (its determinants are not inverted).  FermiNet models created with
+ * JQ_OPT_WITH_GRADIENTS; FP32 arithmetic, FP64 determinant adjoint and FP64
+ * accumulation over walkers. */
+int jq_logpsi_vjp(jq_model* m, const float* elec, int B, const double* cot, float* grad, void* stream);
+/* Floats in the model's parameter pack (the length of jq_logpsi_vjp's grad). */
+size_t jq_model_param_count(const jq_model* m);
+
+/* Device-resident sampler state of jq_sample (MCMCState, sampler/mcmc.py:39-50):
+ * double[3 + adapt_frequency] = {stddev, counter, accepted-move total,
+ * pmoves[adapt_frequency]}.  Initialise to {width, 0, 0, 0...}. */
+#define JQ_MCMC_STATE_LEN(af) (3 + (af))
+
+/* n_outer sampler steps of MCMCSampler.step (sampler/mcmc.py:139-197), all on
+ * the device with no host synchronisation: each recomputes logp = 2 log|psi|
+ * at the current walkers, makes `steps` all-electron moves with the width in
+ * state[0], then updates the state exactly as the reference does (counter += 1,
+ * pmoves[counter % af] = pmove, width x/÷ 1.1 when counter % af == 0 and the
+ * mean pmove leaves [pmove_lo, pmove_hi]).  With n_outer = burn_in this is the
+ * burn-in loop of SamplingStage (workflow/stage/sampling.py:246-301).
+ *   elec (B, N, 3) in/out;  logp (B,) out: 2 log|psi| at the final walkers;
+ *   n_accept (B,) int32 accumulated; state[2] must equal its sum on entry
+ *   (zeroed counts with state[2] = 0);  pmove_out (n_outer,) float64 or NULL.
+ * Philox counters: outer step s, move t uses ctr + s * steps + t. */
+int jq_sample(jq_model* m, float* elec, float* logp, int B, int n_outer, int steps, double* state,
+              int adapt_frequency, double pmove_lo, double pmove_hi, uint64_t seed, uint64_t ctr,
+              int32_t* n_accept, double* pmove_out, void* stream);
+
 /* Mean and E[x^2]-E[x]^2 over the finite entries of x (B,) in FP64, written to
  * out[0], out[1]; out[2] = count of finite entries.  Replaces mean_reduce
  * (estimator/base.py:27-53) on one device (nanmean semantics). */
@@ -120,6 +192,22 @@ int jq_reduce_mean_var(const double* x, int B, double* out, void* stream);
 /* Sum of int32 accept counts (B,) into out[0] (float64).  The pmove numerator
  * of MCMCSampler.step (sampler/mcmc.py:179-180). */
 int jq_reduce_accepts(const int32_t* n_accept, int B, double* out, void* stream);
+
+/* LossAndGrad's loss clipping and gradient weights (estimator/loss_grad.py:
+ * 91-128, utils/clip.py:11-60), FP64 on the device:
+ *   method 0 "mad":  clip to median -+ scale * median(|x - median|)
+ *   method 1 "iqr":  clip to [q1 - scale iqr, q3 + scale iqr]
+ *   method 2 "none"
+ * with NaN-ignoring numpy-"linear" quantiles.  Writes clipped (B,) (or NULL),
+ * weights (B,) = 2 (clipped - mean(clipped)) / n over the n non-NaN walkers
+ * (0 for NaN walkers), so sum_w weights_w grad log|psi_w| is the reference's
+ * loss gradient, and stats[5] = {nanmean(x), nanmean(clipped), lo, hi, n}
+ * (or NULL).  workspace: device memory of jq_loss_clip_workspace_bytes(B)
+ * bytes (partial histograms of the radix select; contents need not be zero). */
+#define JQ_LOSS_CLIP_MAX_B (1 << 28)
+size_t jq_loss_clip_workspace_bytes(int B);
+int jq_loss_clip(const double* x, int B, int method, double scale, double* clipped, double* weights,
+                 double* stats, void* workspace, size_t ws_bytes, void* stream);
 
 const char* jq_last_error(void);
 

@@ -14,13 +14,23 @@ LIB_PATH = pathlib.Path(__file__).resolve().parent / "libjaqmc_amd.so"
 MAX_LAYERS = 8
 
 KIND = {"ferminet": 0, "psiformer": 1}
-ENVELOPE = {"isotropic": 0, "abs_isotropic": 1, "null": 2}
-PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2, "bf16x6": 3}
+ENVELOPE = {"isotropic": 0, "abs_isotropic": 1, "null": 2, "diagonal": 3}
+PRECISION = {"fp32": 0, "bf16": 1, "bf16x3": 2, "bf16x6": 3, "fp16x3": 4}
+LAYER_NORM = {"pre": 0, "post": 1, "null": 2}
+# jq_model_desc.options bits (include/jaqmc_amd.h)
+OPT_NO_ATTN_BIAS = 1 << 0
+OPT_NO_INPUT_BIAS = 1 << 1
+OPT_ORBITAL_BIAS = 1 << 2
+OPT_NO_RESCALE = 1 << 3
+OPT_SHARED_ORBITALS = 1 << 4
+OPT_USE_LAST_LAYER = 1 << 5
+OPT_WITH_GRADIENTS = 1 << 16
 
 EXPORTS = (
     "jq_model_create", "jq_model_destroy", "jq_model_workspace_bytes", "jq_logpsi", "jq_mh_steps",
     "jq_local_energy", "jq_reduce_mean_var", "jq_reduce_accepts", "jq_last_error",
-    "jq_profile_enable", "jq_profile_read",
+    "jq_profile_enable", "jq_profile_read", "jq_orbitals", "jq_sample", "jq_loss_clip",
+    "jq_loss_clip_workspace_bytes", "jq_logpsi_vjp", "jq_model_param_count",
 )
 
 # launch classes of include/jaqmc_amd_prof.h
@@ -48,6 +58,8 @@ class JqModelDesc(ctypes.Structure):
         ("chunk_walkers", ctypes.c_int),
         ("atoms", ctypes.POINTER(ctypes.c_float)),
         ("charges", ctypes.POINTER(ctypes.c_float)),
+        ("layer_norm_mode", ctypes.c_int),
+        ("options", ctypes.c_int),
     ]
 
 
@@ -76,12 +88,22 @@ def load():
     lib.jq_local_energy.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp, vp]
     lib.jq_reduce_mean_var.argtypes = [vp, i32, vp, vp]
     lib.jq_reduce_accepts.argtypes = [vp, i32, vp, vp]
+    lib.jq_orbitals.argtypes = [vp, vp, i32, vp, vp]
+    f64 = ctypes.c_double
+    lib.jq_sample.argtypes = [vp, vp, vp, i32, i32, i32, vp, i32, f64, f64, u64, u64, vp, vp, vp]
+    lib.jq_loss_clip_workspace_bytes.argtypes = [i32]
+    lib.jq_loss_clip_workspace_bytes.restype = sz
+    lib.jq_loss_clip.argtypes = [vp, i32, i32, f64, vp, vp, vp, vp, sz, vp]
+    lib.jq_logpsi_vjp.argtypes = [vp, vp, i32, vp, vp, vp]
+    lib.jq_model_param_count.argtypes = [vp]
+    lib.jq_model_param_count.restype = sz
     lib.jq_last_error.restype = ctypes.c_char_p
     lib.jq_profile_enable.argtypes = [i32]
     lib.jq_profile_read.argtypes = [i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype if name in (
-            "jq_model_workspace_bytes", "jq_last_error") else ctypes.c_int
+            "jq_model_workspace_bytes", "jq_last_error", "jq_loss_clip_workspace_bytes",
+            "jq_model_param_count") else ctypes.c_int
     _lib = lib
     return lib
 

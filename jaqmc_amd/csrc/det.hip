@@ -800,6 +800,7 @@ __global__ __launch_bounds__(64) void combine_kernel(FermiDims d, CombineArgs a)
   const int stride = 3 + (a.lapmode ? T : 0);
   const double* det = a.det + (long)w * K * stride;
   const long gw = (long)a.B_off + w;
+  const long lw = a.lp_local ? (long)w : gw;  // index of the log|psi| output
   const float* x = a.elec + gw * N * 3;
 
   double m = -INFINITY;
@@ -855,7 +856,7 @@ __global__ __launch_bounds__(64) void combine_kernel(FermiDims d, CombineArgs a)
 
   if (!a.lapmode) {
     if (lane == 0) {
-      a.logpsi[gw] = (float)logpsi;
+      a.logpsi[lw] = (float)logpsi;
       if (a.sign) a.sign[gw] = (float)sign;
     }
     return;
@@ -912,7 +913,7 @@ __global__ __launch_bounds__(64) void combine_kernel(FermiDims d, CombineArgs a)
   const double lap = lapdet + wl - gdet2 + jl;
   const double ekin = -0.5 * (lap + g2);
   if (lane == 0) {
-    a.logpsi[gw] = (float)logpsi;
+    a.logpsi[lw] = (float)logpsi;
     if (a.sign) a.sign[gw] = (float)sign;
     a.e_kin[gw] = ekin;
     a.e_pot[gw] = v;
@@ -920,7 +921,33 @@ __global__ __launch_bounds__(64) void combine_kernel(FermiDims d, CombineArgs a)
   }
 }
 
+// Orbitals export (MoleculeWavefunction.orbitals): A[w][k][i][o] =
+// Phi[unit(w, i)][k*N + o] * env[w][i][k*N + o] (value rows, forward mode).
+__global__ void orbital_export_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+                                      const float* __restrict__ env, float* __restrict__ out) {
+  const int KN = d.K * d.N;
+  const long total = (long)d.Bc * d.N * KN;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int ko = (int)(idx % KN);
+    const long wi = idx / KN;
+    const int i = (int)(wi % d.N), w = (int)(wi / d.N);
+    const int k = ko / d.N, o = ko % d.N;
+    const float phi = Phi[(long)unit_idx(d, w, i) * d.Rp * ldphi + ko];
+    out[(((long)w * d.K + k) * d.N + i) * d.N + o] = phi * env[(wi * KN + ko) * 5];
+  }
+}
+
 }  // namespace
+
+int launch_orbital_export(const FermiDims& d, const float* Phi, long ldphi, const float* env, float* out,
+                          hipStream_t s) {
+  const long total = (long)d.Bc * d.N * d.K * d.N;
+  long g = (total + 255) / 256;
+  if (g > 65536L * 8) g = 65536L * 8;
+  hipLaunchKernelGGL(orbital_export_kernel, dim3((unsigned)(g < 1 ? 1 : g)), dim3(256), 0, s, d, Phi, ldphi, env,
+                     out);
+  return 0;
+}
 
 int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* env, double* out, hipStream_t s) {
   if (d.N <= SM_NP) {

@@ -2,6 +2,8 @@
 // the per-mode instantiations (gemm_f32/bf16/x3/x6.hip, kernels in
 // gemm_impl.h), plus a synthetic-buffer timing hook.
 
+#include <vector>
+
 #include "kernels.h"
 
 int g_gemm_variant = 0;
@@ -10,6 +12,7 @@ int gemm_dispatch_f32(const GemmArgs& a, hipStream_t s);
 int gemm_dispatch_bf16(const GemmArgs& a, hipStream_t s);
 int gemm_dispatch_x3(const GemmArgs& a, hipStream_t s);
 int gemm_dispatch_x6(const GemmArgs& a, hipStream_t s);
+int gemm_dispatch_h3(const GemmArgs& a, hipStream_t s);
 void bgemm_dispatch_f32(const BGemmArgs& a, hipStream_t s);
 void bgemm_dispatch_bf16(const BGemmArgs& a, hipStream_t s);
 void bgemm_dispatch_x3(const BGemmArgs& a, hipStream_t s);
@@ -23,7 +26,8 @@ int launch_bgemm(const BGemmArgs& a, int prec, hipStream_t s) {
     case PREC_F32: bgemm_dispatch_f32(a, s); return 0;
     case PREC_BF16: bgemm_dispatch_bf16(a, s); return 0;
     case PREC_BF16X3: bgemm_dispatch_x3(a, s); return 0;
-    case PREC_BF16X6: bgemm_dispatch_x6(a, s); return 0;
+    case PREC_BF16X6:
+    case PREC_FP16X3: bgemm_dispatch_x6(a, s); return 0;
   }
   return -1;
 }
@@ -46,6 +50,8 @@ int launch_gemm(const GemmArgs& a, int prec, bool y_bf16, hipStream_t s) {
     case PREC_BF16: return gemm_dispatch_bf16(a, s);
     case PREC_BF16X3: return gemm_dispatch_x3(a, s);
     case PREC_BF16X6: return gemm_dispatch_x6(a, s);
+    // value-only rows keep the BF16X6 arithmetic (no per-unit row scales)
+    case PREC_FP16X3: return (a.R <= 1 && a.lrp <= 0) ? gemm_dispatch_x6(a, s) : gemm_dispatch_h3(a, s);
   }
   return -1;
 }
@@ -55,22 +61,34 @@ int launch_gemm(const GemmArgs& a, int prec, bool y_bf16, hipStream_t s) {
 // g_gemm_variant; variant >= 100 selects the plain row-tile-major order.
 extern "C" int jq_bench_gemm(int prec, int R, int rows, int K, int ncols, int act, int variant, int reps,
                              float* ms_out) {
-  if (rows <= 0 || K <= 0 || ncols <= 0 || K % 32 || (R > 1 && rows % R) || reps <= 0) return -1;
-  const int ns = prec == PREC_BF16X6 ? 3 : (prec == PREC_BF16X3 ? 2 : 1);
+  if (rows <= 0 || K <= 0 || ncols <= 0 || K % 32 || (R > 1 && rows % R) || reps <= 0 || prec < 0 ||
+      prec > PREC_FP16X3)
+    return -1;
+  // weight planes: NS BF16 planes; FP16X3 adds two FP16 planes and 2*ncols
+  // FP32 column scales after three BF16 planes
+  const size_t wbytes = prec == PREC_FP16X3 ? (size_t)ncols * K * 5 * 2 + (size_t)ncols * 8
+                                            : (size_t)ncols * K * 2 * (prec == PREC_BF16X6 ? 3 : (prec == PREC_BF16X3 ? 2 : 1));
+  // G holds one (R x ncols) block per walker of the unit -> walker map below
+  // (7 + 7 electrons): unit u belongs to a walker below units / 7 + 1
+  const long units = R > 1 ? rows / R : rows;
+  const size_t gbytes = (size_t)ncols * 4 * (R > 1 ? R : 1) * (units / 7 + 2);
   float *X = nullptr, *W = nullptr, *Y = nullptr, *G = nullptr, *b = nullptr;
   bf16* Wb = nullptr;
   const long nW = (long)ncols * K;
   int rc = 0;
   if (hipMalloc(&X, (size_t)rows * K * 4) || hipMalloc(&W, nW * 4) || hipMalloc(&Y, (size_t)rows * ncols * 4) ||
-      hipMalloc(&Wb, nW * ns * 2) || hipMalloc(&G, (size_t)ncols * 4 * (R > 1 ? R : 1) * (rows / (R > 1 ? R : 1) / 14 + 1)) ||
-      hipMalloc(&b, ncols * 4))
+      hipMalloc(&Wb, wbytes) || hipMalloc(&G, gbytes) || hipMalloc(&b, ncols * 4))
     rc = -3;
   if (!rc) {
     (void)hipMemset(X, 0, (size_t)rows * K * 4);
     (void)hipMemset(W, 0, nW * 4);
-    (void)hipMemset(Wb, 0, nW * ns * 2);
+    (void)hipMemset(Wb, 0, wbytes);
     (void)hipMemset(b, 0, ncols * 4);
-    (void)hipMemset(G, 0, (size_t)ncols * 4 * (R > 1 ? R : 1) * (rows / (R > 1 ? R : 1) / 14 + 1));
+    (void)hipMemset(G, 0, gbytes);
+    if (prec == PREC_FP16X3) {  // unit column scales
+      std::vector<float> ones(2 * (size_t)ncols, 1.f);
+      (void)hipMemcpy((char*)Wb + (size_t)ncols * K * 10, ones.data(), ones.size() * 4, hipMemcpyHostToDevice);
+    }
     GemmArgs a{};
     a.X = X;
     a.ldx = K;
@@ -88,16 +106,15 @@ extern "C" int jq_bench_gemm(int prec, int R, int rows, int K, int ncols, int ac
     a.act = act ? ACT_TANH : ACT_NONE;
     a.res_scale = 1.f;
     if (act == 2 && ncols <= K) {  // realistic layer epilogue: G broadcast + residual
-      const int units = R > 1 ? rows / R : rows;
-      a.Bc = units / 14 > 0 ? units / 14 : 1;
+      a.Bc = units / 14 > 0 ? (int)(units / 14) : 1;
       a.n_up = 7;
       a.n_dn = 7;
       a.G = G;
       a.ldg = ncols;
       a.Res = X;
       a.ldres = K;
-      if ((long)a.Bc * (R > 1 ? R : 1) > 8192L * (R > 1 ? R : 1)) rc = -1;
     }
+    if (R <= 1) a.lrp = 48;  // plain rows laid out as 48-row tangent units
     a.linear_tiles = variant >= 100 ? 1 : 0;
     const int old = g_gemm_variant;
     g_gemm_variant = variant % 100;

@@ -17,8 +17,19 @@
 //   PREC_BF16X6  x = h + m + l (three BF16 terms): hh + hm + mh + hl + lh + mm,
 //                products down to 2^-16 relative are kept, so the result has
 //                FP32-level error (~2^-24) with FP32 accumulation.
-// Activations are split into BF16 terms when staged into LDS; weights are
-// constant and arrive pre-split as NS BF16 planes (GemmArgs::Wt_lo).
+//   PREC_FP16X3  x = h + l (two FP16 terms, 11-bit significands): hh + hl + lh
+//                with v_mfma_f32_16x16x32_f16, operand error ~2^-22.  FP16's
+//                range is handled with exact power-of-two scales: each weight
+//                column is scaled on upload so its largest entry lies in
+//                [2^13, 2^14) (the low terms stay normal); the value and
+//                Laplacian rows of each unit are scanned before the K loop and
+//                scaled the same way; Jacobian rows take the fixed scale 2^6
+//                (entries below 2^10 in magnitude stay finite, and their
+//                absolute error floor is 2^-31).  The accumulators are unscaled
+//                before the epilogue.  Value-only launches (no tangent rows)
+//                use the BF16X6 kernel.
+// Activations are split into 16-bit terms when staged into LDS; weights are
+// constant and arrive pre-split as NS planes (GemmArgs::Wt_lo).
 //
 // K-to-lane mapping: within a 32-wide K chunk, lane group g = lane/16 owns
 // k = 8g .. 8g+7.  For the BF16 16x16x32 MFMA this is one 16-byte LDS read per
@@ -41,15 +52,20 @@ namespace {
 template <int PREC> struct NSplit { static constexpr int v = 1; };
 template <> struct NSplit<PREC_BF16X3> { static constexpr int v = 2; };
 template <> struct NSplit<PREC_BF16X6> { static constexpr int v = 3; };
+template <> struct NSplit<PREC_FP16X3> { static constexpr int v = 2; };
 
-__device__ inline void split3(float x, bf16* t, int ns) {
+template <typename T>
+__device__ inline void split3(float x, T* t, int ns) {
   float r = x;
   for (int i = 0; i < ns; ++i) {
-    const bf16 h = (bf16)r;
+    const T h = (T)r;
     t[i] = h;
     r = r - (float)h;
   }
 }
+
+// Fixed power-of-two scale of Jacobian rows in PREC_FP16X3 (see above).
+constexpr float kF16JScale = 64.f;
 
 // MBU: 16-row blocks per unit (Laplacian mode); UPT: units per tile; NBW:
 // 16-column blocks per wave.  BM = 16*MBU*UPT, BN = 64*NBW.
@@ -67,9 +83,10 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   constexpr int BN = 64 * NBW;
   constexpr int KC = 32;
   constexpr bool F32 = PREC == PREC_F32;
+  constexpr bool F16 = PREC == PREC_FP16X3;
   constexpr int NS = NSplit<PREC>::v;
   constexpr int LS = F32 ? 36 : 40;  // LDS row stride (elements)
-  using TL = typename std::conditional<F32, float, bf16>::type;
+  using TL = typename std::conditional<F32, float, typename std::conditional<F16, f16, bf16>::type>::type;
   constexpr int NSL = F32 ? 1 : NS;
   // LDS holds the activation tile of one K chunk, which all four waves read;
   // the epilogue reuses it to stage output rows.  The weights are not staged:
@@ -81,6 +98,8 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   constexpr int ARENA = A_BYTES > STG_BYTES ? A_BYTES : STG_BYTES;
   __shared__ __attribute__((aligned(16))) char arena[ARENA];
   TL (*As)[BM * LS] = (TL (*)[BM * LS])arena;
+  // PREC_FP16X3: power-of-two scale of each tile row (1 in the other modes)
+  __shared__ float rsc[F16 ? BM : 1], rsi[F16 ? BM : 1];
 
   // register prefetch: A chunk as float4 (8 per 32-wide row); B fragments
   constexpr int AV = (BM * 8 + 255) / 256;
@@ -105,7 +124,10 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   const int col0 = cb * BN;
   const float* X = (const float*)a.X + (long)blockIdx.z * a.sxb;
   const float* W = (const float*)a.Wt;
-  const bf16* Wb = (const bf16*)a.Wt_lo;
+  // FP16X3 weights: two FP16 planes after the three BF16 planes, then the
+  // inverse column scales and the column scales (powers of two)
+  const TL* Wb = F16 ? (const TL*)((const bf16*)a.Wt_lo + 3 * a.wplane) : (const TL*)a.Wt_lo;
+  const float* csinv = F16 ? (const float*)(Wb + 2 * a.wplane) : nullptr;
 
   auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma unroll
@@ -142,19 +164,70 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       if constexpr (F32) {
         *(float4*)(&As[0][r * LS + q * 4]) = pa[v];
       } else {
-        bf16 t[4][NS];
-        split3(pa[v].x, t[0], NS);
-        split3(pa[v].y, t[1], NS);
-        split3(pa[v].z, t[2], NS);
-        split3(pa[v].w, t[3], NS);
+        float4 x = pa[v];
+        if constexpr (F16) {
+          const float sc = rsc[r];
+          x.x *= sc;
+          x.y *= sc;
+          x.z *= sc;
+          x.w *= sc;
+        }
+        TL t[4][NS];
+        split3(x.x, t[0], NS);
+        split3(x.y, t[1], NS);
+        split3(x.z, t[2], NS);
+        split3(x.w, t[3], NS);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-          bf16 h[4] = {t[0][s], t[1][s], t[2][s], t[3][s]};
+          TL h[4] = {t[0][s], t[1][s], t[2][s], t[3][s]};
           *(uint2*)(&As[s][r * LS + q * 4]) = *(uint2*)h;
         }
       }
     }
   };
+
+  // PREC_FP16X3 row scales: the value and Laplacian rows of every unit
+  // (row % period 0 or 1) are scanned over K (one wave per row); other rows
+  // take the fixed Jacobian scale.
+  float csi[NBW], csf[NBW];  // inverse and plain column scales of the lane's columns (FP16X3)
+  if constexpr (F16) {
+    const int per = LAP ? 16 * MBU : (a.lrp > 0 ? a.lrp : 1);
+    for (int r = wave; r < BM; r += 4) {
+      const long gr = row0 + r;
+      const int cls = (int)(gr % per);
+      float sc = kF16JScale, si = 1.f / kF16JScale;
+      if (gr >= a.rows) {
+        sc = si = 1.f;
+      } else if (cls <= 1) {
+        float mx = 0.f;
+        for (int k = lane * 4; k < a.K; k += 256) {
+          const float4 v = *(const float4*)(X + gr * a.ldx + k);
+          mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        }
+        mx = wave_max(mx);
+        sc = si = 1.f;
+        if (mx > 0.f && mx <= 3.0e38f) {
+          int e;
+          (void)frexpf(mx, &e);
+          int sh = 14 - e;
+          sh = sh < -60 ? -60 : (sh > 60 ? 60 : sh);
+          sc = ldexpf(1.f, sh);
+          si = ldexpf(1.f, -sh);
+        }
+      }
+      if (lane == 0) {
+        rsc[r] = sc;
+        rsi[r] = si;
+      }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NBW; ++nb) {
+      const int gc = col0 + wcol + nb * 16 + lr;
+      csi[nb] = gc < a.ncols ? csinv[gc] : 1.f;
+      csf[nb] = gc < a.ncols ? csinv[a.ncols + gc] : 1.f;
+    }
+    __syncthreads();
+  }
 
   f32x4 acc[MB][NBW];
 #pragma unroll
@@ -187,10 +260,12 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const float* gr = gu + (long)(m * 16 + 4 * lg + v) * a.ldg;
+              // FP16X3: the accumulators carry the row and column scales
+              const float gs = F16 ? rsc[u * 16 * MBU + m * 16 + 4 * lg + v] : 1.f;
 #pragma unroll
               for (int nb = 0; nb < NBW; ++nb) {
                 const float g = gr[ncl[nb]];
-                acc[u * MBU + m][nb][v] = cin[nb] ? g : 0.f;
+                acc[u * MBU + m][nb][v] = cin[nb] ? (F16 ? g * gs * csf[nb] : g) : 0.f;
               }
             }
             // at most four row blocks of loads in flight (register pressure)
@@ -265,6 +340,15 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         for (int s = 0; s < NS; ++s) at[s] = *(const bf16x8*)&As[s][(mb * 16 + lr) * LS + lg * 8];
 #pragma unroll
         for (int nb = 0; nb < NBW; ++nb) {
+          if constexpr (F16) {
+            // small terms first
+            const f16x8 a0 = __builtin_bit_cast(f16x8, at[0]), a1 = __builtin_bit_cast(f16x8, at[1]);
+            const f16x8 b0 = __builtin_bit_cast(f16x8, bt[0][nb]), b1 = __builtin_bit_cast(f16x8, bt[1][nb]);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b0, acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b1, acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b0, acc[mb][nb], 0, 0, 0);
+            continue;
+          }
           // small terms first
           if constexpr (NS == 3) {
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[1][nb], acc[mb][nb], 0, 0, 0);
@@ -281,6 +365,17 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       if (it + 1 < nk) bload(kof(it + 1));
     }
     __syncthreads();
+  }
+
+  if constexpr (F16) {  // remove the power-of-two row and column scales
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float ir = rsi[mb * 16 + 4 * lg + v];
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) acc[mb][nb][v] *= ir * csi[nb];
+      }
   }
 
   // ---- epilogue (registers) ----
@@ -389,7 +484,16 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     for (int it = 0; it < 16 / RPI; ++it) {
       const int r = r_lane + RPI * it;
       const long grow = row0 + mb0 * 16 + r;
-      if (grow >= a.rows || !nok) continue;
+      if (grow >= a.rows) continue;
+      if (!nok) {  // padding columns [ncols, zpad) are written as zeros
+        if (n < a.zpad && a.hs_H == 0) {
+          float* dz = Y + grow * a.ldy + n;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (n + q < a.zpad) dz[q] = 0.f;
+        }
+        continue;
+      }
       float y[4];
       *(float4*)y = *(const float4*)&stg[r * SW + c4];
       if (!LAP) {
@@ -447,8 +551,12 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         *(float4*)dst = *(float4*)y;
       } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (n + q < a.ncols) dst[q] = y[q];
+        for (int q = 0; q < 4; ++q) {
+          if (n + q < a.ncols)
+            dst[q] = y[q];
+          else if (n + q < a.zpad && a.hs_H == 0)
+            dst[q] = 0.f;
+        }
       }
     }
     wave_sync();

@@ -3,7 +3,7 @@
 
 #include "common.h"
 
-enum { PREC_F32 = 0, PREC_BF16 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
+enum { PREC_F32 = 0, PREC_BF16 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3, PREC_FP16X3 = 4 };
 enum { ACT_NONE = 0, ACT_TANH = 1 };
 
 // Y[rows, ncols] = epilogue(X[rows, K] . Wt[ncols, K]^T)
@@ -20,8 +20,11 @@ enum { ACT_NONE = 0, ACT_TANH = 1 };
 struct GemmArgs {
   const void* X; long ldx;
   const void* Wt; long ldw;        // FP32 (ncols, K) transposed weight (PREC_F32)
-  const void* Wt_lo; long wplane;  // BF16 modes: NS BF16 planes of the split weight,
-                                   // plane s at Wt_lo + s * wplane (elements)
+  const void* Wt_lo; long wplane;  // split modes: NS BF16 planes of the split weight,
+                                   // plane s at Wt_lo + s * wplane (elements).  PREC_FP16X3
+                                   // weights hold three BF16 planes (used by the value-only
+                                   // launches), then two FP16 planes of the column-scaled
+                                   // weight and the ncols inverse column scales (FP32)
   void* Y; long ldy;
   int rows, K, ncols;
   const float* bias;   // (ncols) or null
@@ -38,6 +41,11 @@ struct GemmArgs {
   // (w, i), row r goes to Y[(((w*3 + part)*H + h)*N + i)*R*dh + r*dh + d]
   int hs_H, hs_dh, hs_D, hs_N;
   int linear_tiles;  // 1: plain row-tile-major workgroup order (no XCD grouping)
+  int lrp;           // rows are tangent rows of units of lrp rows (value, Laplacian, Jacobian...);
+                     // 0: value rows only.  PREC_FP16X3 scales the value and Laplacian rows
+                     // of each unit by powers of two before the FP16 split (set by the caller
+                     // for R = 1 launches on Laplacian-mode data; R > 1 implies lrp = R)
+  int zpad;          // > ncols: columns [ncols, zpad) of Y are written as zeros
 };
 
 // Batched GEMM on activation operands (both FP32 in HBM, split on load for the
@@ -102,21 +110,32 @@ struct CombineArgs {
   int jastrow; float alpha_par, alpha_anti;
   float* logpsi; float* sign; double* e_kin; double* e_pot; double* e_loc; float* grad;
   int lapmode;
+  int lp_local;  // logpsi indexed by chunk walker (model scratch) instead of B_off + walker
 };
 int launch_combine(const FermiDims& d, const CombineArgs& a, hipStream_t s);
 
 // ---- Metropolis (mcmc.hip) ----
-int launch_propose(const float* elec, float* prop, int B, int N, float width, const float* normals,
-                   uint64_t seed, uint64_t ctr, hipStream_t s);
+// Walkers [w0, w0 + B) of a batch: elec/prop/normals/uniforms/logp/n_accept
+// point at walker w0; the Philox streams are keyed by the global index w0 + w.
+// width_dev (device, the sampler state's stddev as FP64) overrides width when set.
+int launch_propose(const float* elec, float* prop, int B, int N, float width, const double* width_dev,
+                   const float* normals, uint64_t seed, uint64_t ctr, int w0, hipStream_t s);
 int launch_accept(float* elec, const float* prop, float* logp, const float* logpsi_prop, int B, int N,
-                  const float* uniforms, uint64_t seed, uint64_t ctr, int32_t* n_accept, hipStream_t s);
+                  const float* uniforms, uint64_t seed, uint64_t ctr, int w0, int32_t* n_accept, hipStream_t s);
+// MCMCSampler.step's adaptation on the device state (see jq_sample).
+int launch_mcmc_adapt(double* state, const int32_t* n_accept, int B, int steps, int af, double lo, double hi,
+                      double* pmove_out, hipStream_t s);
+// A[w][k][i][o] = Phi value row * envelope value (orbitals export).
+int launch_orbital_export(const FermiDims& d, const float* Phi, long ldphi, const float* env, float* out,
+                          hipStream_t s);
 int launch_reduce_mean_var(const double* x, int B, double* out, hipStream_t s);
 int launch_reduce_accepts(const int32_t* x, int B, double* out, hipStream_t s);
 int launch_scale_f32(float* x, int n, float a, hipStream_t s);
 
 // ---- Psiformer kernels (psiformer.hip) ----
-// Layer-0 rescaled atom-electron features + spin column into X[:, :, 0:kin).
-int launch_psi_features(const FermiDims& d, const float* elec, const float* atoms, float* X, int kin,
+// Layer-0 atom-electron features (rescaled [log(1+r), d log(1+r)/r] when
+// rescale, else [r, d]) + spin column into X[:, :, 0:kin).
+int launch_psi_features(const FermiDims& d, const float* elec, const float* atoms, float* X, int kin, int rescale,
                         hipStream_t s);
 // LayerNorm forward Laplacian over the feature axis, one unit (Rp rows) per
 // workgroup: y = (x - mean) rsqrt(var + eps) * scale + bias.

@@ -45,8 +45,10 @@ constexpr uint32_t TAG_NORMAL = 0u;
 constexpr uint32_t TAG_UNIFORM = 0x80000000u;
 
 __global__ void propose_kernel(const float* __restrict__ elec, float* __restrict__ prop, int B, int N3, float width,
-                               const float* __restrict__ normals, uint64_t seed, uint64_t step) {
+                               const double* __restrict__ width_dev, const float* __restrict__ normals,
+                               uint64_t seed, uint64_t step, int w0) {
   // one thread per group of 4 coordinates of one walker
+  if (width_dev) width = (float)width_dev[0];
   const int groups = (N3 + 3) / 4;
   const long total = (long)B * groups;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
@@ -59,8 +61,9 @@ __global__ void propose_kernel(const float* __restrict__ elec, float* __restrict
         z[e] = c < N3 ? normals[(long)w * N3 + c] : 0.f;
       }
     } else {
-      const u32x4 r = philox4x32_10(u32x4{(uint32_t)gi, (uint32_t)w, (uint32_t)step, (uint32_t)(step >> 32) ^ TAG_NORMAL},
-                                    (uint32_t)seed, (uint32_t)(seed >> 32));
+      const u32x4 r = philox4x32_10(
+          u32x4{(uint32_t)gi, (uint32_t)(w0 + w), (uint32_t)step, (uint32_t)(step >> 32) ^ TAG_NORMAL},
+          (uint32_t)seed, (uint32_t)(seed >> 32));
       const float u1 = u01(r.x), u2 = u01(r.y), u3 = u01(r.z), u4 = u01(r.w);
       const float m1 = sqrtf(-2.f * logf(u1)), m2 = sqrtf(-2.f * logf(u3));
       float s1, c1, s2, c2;
@@ -80,7 +83,7 @@ __global__ void propose_kernel(const float* __restrict__ elec, float* __restrict
 
 __global__ void accept_kernel(float* __restrict__ elec, const float* __restrict__ prop, float* __restrict__ logp,
                               const float* __restrict__ logpsi_prop, int B, int N3, const float* __restrict__ uniforms,
-                              uint64_t seed, uint64_t step, int32_t* __restrict__ n_accept) {
+                              uint64_t seed, uint64_t step, int w0, int32_t* __restrict__ n_accept) {
   const int w = blockIdx.x;
   __shared__ int acc_s;
   if (threadIdx.x == 0) {
@@ -88,8 +91,9 @@ __global__ void accept_kernel(float* __restrict__ elec, const float* __restrict_
     if (uniforms) {
       u = uniforms[w];
     } else {
-      const u32x4 r = philox4x32_10(u32x4{0u, (uint32_t)w, (uint32_t)step, (uint32_t)(step >> 32) ^ TAG_UNIFORM},
-                                    (uint32_t)seed, (uint32_t)(seed >> 32));
+      const u32x4 r = philox4x32_10(
+          u32x4{0u, (uint32_t)(w0 + w), (uint32_t)step, (uint32_t)(step >> 32) ^ TAG_UNIFORM},
+          (uint32_t)seed, (uint32_t)(seed >> 32));
       u = u01(r.x);
     }
     const float lp2 = 2.f * logpsi_prop[w];
@@ -156,6 +160,42 @@ __global__ __launch_bounds__(1024) void sum_i32_kernel(const int32_t* __restrict
   }
 }
 
+// MCMCSampler.step's bookkeeping after `steps` moves (sampler/mcmc.py:179-195)
+// on the device state {stddev, counter, accepted total, pmoves[af]}:
+// pmove = (sum of n_accept - previous total) / (steps * B); counter += 1;
+// pmoves[counter % af] = pmove; when counter % af == 0 the width is multiplied
+// by 1.1 if mean(pmoves) > hi, divided by 1.1 if < lo.  One workgroup.
+__global__ __launch_bounds__(1024) void mcmc_adapt_kernel(double* __restrict__ st, const int32_t* __restrict__ n_accept,
+                                                          int B, int steps, int af, double lo, double hi,
+                                                          double* __restrict__ pmove_out) {
+  __shared__ double s1[16];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < B; i += blockDim.x) a += (double)n_accept[i];
+  a = wave_sum_d(a);
+  if ((threadIdx.x & 63) == 0) s1[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) tot += s1[i];
+    const double pmove = (tot - st[2]) / ((double)steps * (double)B);
+    st[2] = tot;
+    const double counter = st[1] + 1.0;
+    st[1] = counter;
+    const long t = (long)counter % af;
+    st[3 + t] = pmove;
+    if (t == 0) {
+      double m = 0.0;
+      for (int i = 0; i < af; ++i) m += st[3 + i];
+      m /= af;
+      if (m > hi)
+        st[0] = st[0] * 1.1;
+      else if (m < lo)
+        st[0] = st[0] / 1.1;
+    }
+    if (pmove_out) pmove_out[0] = pmove;
+  }
+}
+
 __global__ void scale_kernel(float* x, int n, float a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] *= a;
@@ -163,18 +203,25 @@ __global__ void scale_kernel(float* x, int n, float a) {
 
 }  // namespace
 
-int launch_propose(const float* elec, float* prop, int B, int N, float width, const float* normals, uint64_t seed,
-                   uint64_t step, hipStream_t s) {
+int launch_propose(const float* elec, float* prop, int B, int N, float width, const double* width_dev,
+                   const float* normals, uint64_t seed, uint64_t step, int w0, hipStream_t s) {
   const long total = (long)B * ((3 * N + 3) / 4);
   const int grid = (int)((total + 255) / 256);
-  hipLaunchKernelGGL(propose_kernel, grid, 256, 0, s, elec, prop, B, 3 * N, width, normals, seed, step);
+  hipLaunchKernelGGL(propose_kernel, grid, 256, 0, s, elec, prop, B, 3 * N, width, width_dev, normals, seed, step,
+                     w0);
   return 0;
 }
 
 int launch_accept(float* elec, const float* prop, float* logp, const float* logpsi_prop, int B, int N,
-                  const float* uniforms, uint64_t seed, uint64_t step, int32_t* n_accept, hipStream_t s) {
-  hipLaunchKernelGGL(accept_kernel, B, 64, 0, s, elec, prop, logp, logpsi_prop, B, 3 * N, uniforms, seed, step,
+                  const float* uniforms, uint64_t seed, uint64_t step, int w0, int32_t* n_accept, hipStream_t s) {
+  hipLaunchKernelGGL(accept_kernel, B, 64, 0, s, elec, prop, logp, logpsi_prop, B, 3 * N, uniforms, seed, step, w0,
                      n_accept);
+  return 0;
+}
+
+int launch_mcmc_adapt(double* state, const int32_t* n_accept, int B, int steps, int af, double lo, double hi,
+                      double* pmove_out, hipStream_t s) {
+  hipLaunchKernelGGL(mcmc_adapt_kernel, 1, 1024, 0, s, state, n_accept, B, steps, af, lo, hi, pmove_out);
   return 0;
 }
 

@@ -2,12 +2,31 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/jaqmc_amd.h"
 #include "kernels.h"
 
 int jq_fail(int code, const std::string& msg);
+// Per-device stream for model set-up and teardown (api.hip).
+hipStream_t jq_setup_stream();
+
+// Sequential reader over the flat FP32 parameter pack.
+struct PackReader {
+  const float* p;
+  size_t n, pos = 0;
+  bool ok = true;
+  const float* take(size_t k) {
+    if (pos + k > n) {
+      ok = false;
+      return nullptr;
+    }
+    const float* r = p + pos;
+    pos += k;
+    return r;
+  }
+};
 
 // One FermiNet layer (FermiLayers, wavefunction/backbone/ferminet.py:31-63).
 // Envelope parameters arrive as (orbital o, atom I, det k) (the Flax
@@ -33,6 +52,11 @@ struct FermiLayer {
   float* b = nullptr;                    // (d1)
   float* W2 = nullptr;                   // (f2, d2) row-major FP32 (pair stream)
   float* b2 = nullptr;
+  // parameter-gradient pass: FP32 weights in their original (in, out)
+  // orientation, padded to K = d1 columns, and pack offsets
+  float* Wm_o = nullptr;                 // (kmain, d1): rows [h | pair means]
+  float* Wg_o = nullptr;                 // (kg, d1): rows [spin means]
+  long off_w = 0, off_b = 0, off_w2 = 0, off_b2 = 0;
 };
 
 // One Psiformer layer (PsiformerLayer, wavefunction/backbone/psiformer.py:64-98).
@@ -67,12 +91,35 @@ struct jq_model {
   void *Win = nullptr, *Win_lo = nullptr;
   float* bin = nullptr;
 
-  // output layers
+  // output layers (with orbitals_spin_split=False both channel slots point at
+  // the one shared kernel / envelope)
   int korb = 0;  // padded K of the orbital GEMM
+  int norb = 1;  // orbital kernels in the pack: nch, or 1 when shared
   void* Worb[2] = {nullptr, nullptr};
   void* Worb_lo[2] = {nullptr, nullptr};
+  float* borb[2] = {nullptr, nullptr};  // orbital bias (bias_orbitals) or null
   float* pi[2] = {nullptr, nullptr};
   float* sigma[2] = {nullptr, nullptr};
+  int ln_mode = JQ_LN_PRE;  // Psiformer LayerNorm placement
+  int options = 0;          // JQ_OPT_* bits
+
+  // parameter-gradient pass (FermiNet, created with JQ_OPT_WITH_GRADIENTS)
+  bool grads = false;
+  int gchunk = 0;           // walkers per gradient chunk
+  long n_params = 0;        // floats in the parameter pack
+  long off_orb[2] = {0, 0}, off_env[2] = {0, 0};
+  float* Worb_o[2] = {nullptr, nullptr};  // (d_last, KNpad) original orientation
+  long ldko = 0;                          // KN padded to 32
+  std::vector<void*> Xt;                  // tape: layer inputs X_0 .. X_L (value rows)
+  std::vector<float*> H2t;                // tape: pair stream h2_0 .. h2_{L-1}
+  std::vector<void*> gt;                  // tape: spin means g_0 .. g_{L-1}
+  float *dH[2] = {nullptr, nullptr}, *dZ = nullptr, *dX = nullptr, *dG = nullptr, *dg = nullptr;
+  float *dH2[2] = {nullptr, nullptr}, *dZ2 = nullptr, *dPhi = nullptr, *dEnv = nullptr, *Pw = nullptr;
+  long pw_cap = 0, ldd = 0;
+  double* gacc = nullptr;                 // FP64 gradient accumulator (n_params)
+  int alloc_grad_workspace(size_t tape_budget);
+  int forward_ferminet_tape(const float* elec_chunk, int Bc, hipStream_t s);
+  int vjp(const float* elec, int B, const double* cot, float* grad, hipStream_t s);
 
   // workspace (sized for `chunk` walkers in Laplacian mode; forward mode reuses
   // it for `chunk_fwd` walkers)
@@ -90,26 +137,34 @@ struct jq_model {
   // Psiformer scratch
   void* P[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   float* stats = nullptr;
-  // Metropolis scratch (grown on demand)
+  // Metropolis scratch, allocated at create: proposals of one forward chunk
+  // and log|psi| of one chunk
   float* prop = nullptr;
   float* lp = nullptr;
-  int prop_cap = 0;
+  bool skip_det = false;  // orbitals export: stop after Phi and the envelope
+  // streams this model has been used on, each with an event recorded after the
+  // model's last call there (destroy waits for these, nothing device-wide)
+  std::vector<std::pair<hipStream_t, hipEvent_t>> used;
+  void touch(hipStream_t s);
 
   ~jq_model();
   int init(const jq_model_desc& desc, const float* packed, size_t n_floats);
   int init_ferminet(const float* packed, size_t n_floats);
   int init_psiformer(const float* packed, size_t n_floats);
-  int ensure_scratch(int B);
+  int alloc_mh_scratch();
   // Run one walker chunk [b0, b0 + Bc) of `elec` (global walker indexing) and
   // write per-walker outputs at offset b0.
+  // logpsi is indexed by global walker (b0 + w), or by chunk walker w when
+  // lp_local (the model's own scratch)
   int forward(const float* elec, int b0, int Bc, bool lap, float* logpsi, float* sign, double* e_kin,
-              double* e_pot, double* e_loc, float* grad, hipStream_t s);
+              double* e_pot, double* e_loc, float* grad, hipStream_t s, bool lp_local = false);
   int forward_ferminet(const float* elec_chunk, int Bc, bool lap, hipStream_t s);
   int forward_psiformer(const float* elec_chunk, int Bc, bool lap, hipStream_t s);
   int orbitals_and_dets(const float* elec_chunk, const void* Xlast, int Bc, bool lap, hipStream_t s);
 
+  int read_outputs(PackReader& rd, int d_last);
   int upload(const void* host, size_t bytes, void** dev);
-  int upload_wt(const std::vector<float>& wt, void** hi, void** lo);
+  int upload_wt(const std::vector<float>& wt, int ncols, void** hi, void** lo);
   void* alloc(size_t bytes);
   size_t es() const { return xbf ? 2 : 4; }
   FermiDims dims(int Bc, bool lap) const {

@@ -23,8 +23,10 @@ inline int grid_for(long total) {
 // Derivatives wrt x_i only (Local1): with q = g / r,
 //   grad g = g' d / r,  lap g = g'' + 2 g'/r
 //   d/dx_b (d_a q) = delta_ab q + d_a q' d_b / r,  lap (d_a q) = d_a (q'' + 2 q'/r) + 2 q' d_a / r.
+// With rescale = 0 (MoleculeFeatures(rescale=False)) the features are
+// [r, d] as in FermiNet: grad r = d / r, lap r = 2 / r; grad d_a = e_a, lap d_a = 0.
 __global__ void psi_features_kernel(FermiDims d, const float* __restrict__ elec, const float* __restrict__ atoms,
-                                    float* __restrict__ X, int kin) {
+                                    float* __restrict__ X, int kin, int rescale) {
   const long total = (long)d.Bc * d.N * d.Rp * kin;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
     const int f = idx % kin;
@@ -46,7 +48,22 @@ __global__ void psi_features_kernel(FermiDims d, const float* __restrict__ elec,
       const float dv[3] = {xi[0] - atoms[3 * I], xi[1] - atoms[3 * I + 1], xi[2] - atoms[3 * I + 2]};
       const float rr = sqrtf(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
       const float g = log1pf(rr), g1 = 1.f / (1.f + rr), g2 = -g1 * g1;
-      if (e == 0) {
+      if (!rescale) {
+        const int b = (r - ROW_J0) % 3;
+        if (e == 0) {
+          if (r == ROW_VAL)
+            out = rr;
+          else if (r == ROW_LAP)
+            out = 2.f / rr;
+          else
+            out = dv[b] / rr;
+        } else {
+          if (r == ROW_VAL)
+            out = dv[e - 1];
+          else if (r != ROW_LAP)
+            out = b == e - 1 ? 1.f : 0.f;
+        }
+      } else if (e == 0) {
         if (r == ROW_VAL)
           out = g;
         else if (r == ROW_LAP)
@@ -403,9 +420,10 @@ __global__ __launch_bounds__(256) void softmax_lap_kernel(SoftmaxArgs a) {
 
 }  // namespace
 
-int launch_psi_features(const FermiDims& d, const float* elec, const float* atoms, float* X, int kin, hipStream_t s) {
+int launch_psi_features(const FermiDims& d, const float* elec, const float* atoms, float* X, int kin, int rescale,
+                        hipStream_t s) {
   const long total = (long)d.Bc * d.N * d.Rp * kin;
-  hipLaunchKernelGGL(psi_features_kernel, grid_for(total), 256, 0, s, d, elec, atoms, X, kin);
+  hipLaunchKernelGGL(psi_features_kernel, grid_for(total), 256, 0, s, d, elec, atoms, X, kin, rescale);
   return 0;
 }
 

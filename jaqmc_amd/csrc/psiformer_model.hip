@@ -2,7 +2,9 @@
 // wavefunction/backbone/psiformer.py:29-187): weight packing and the
 // per-chunk forward / forward-Laplacian sequence.
 //
-// Per layer (pre-LN):  x += Wo . MHA(LN1(x));  x += tanh(W2 tanh(W1 LN2(x)))
+// Per layer, pre-LN (the default):  x += Wo . MHA(LN1(x));  x += MLP(LN2(x));
+// post-LN:  x = LN1(x + Wo . MHA(x));  x = LN2(x + MLP(x));  null: no LayerNorm
+// (backbone/psiformer.py:64-98); MLP(y) = tanh(W2 tanh(W1 y + b1) + b2).
 // Attention per (walker w, head h), q pre-scaled by 1/sqrt(dh) in the weights:
 //   S1[(i,c)][j] = sum_d Q[i][c][d] K[j][0][d]          (Q Jacobian/lap rows x key values)
 //   S2[(j,c)][i] = sum_d K[j][c][d] Q[i][0][d]
@@ -22,20 +24,11 @@
 
 
 namespace {
-struct Reader {
-  const float* p;
-  size_t n, pos = 0;
-  bool ok = true;
-  const float* take(size_t k) {
-    if (pos + k > n) {
-      ok = false;
-      return nullptr;
-    }
-    const float* r = p + pos;
-    pos += k;
-    return r;
-  }
-};
+bool all_zero(const float* p, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (p[i] != 0.f) return false;
+  return true;
+}
 
 // dst (ncols x kpad) at column offset c0 <- transposed rows [r0, r0+nr) of W
 // (rows x ldw_in, taking columns [cin0, cin0 + ncols)), scaled.
@@ -57,16 +50,18 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
   if (L < 1 || H < 1 || dh < 4 || dh % 32 != 0 || d.n_mlp < 0 || d.n_mlp > 3 || (D != 64 && D != 128 && D != 256 && D != 512))
     return jq_fail(JQ_EINVAL, "unsupported Psiformer shape (heads_dim a multiple of 32, model dim 64/128/256/512)");
   if (Rp > 128) return jq_fail(JQ_EINVAL, "Psiformer supports N <= 42 electrons (tangent tile <= 128 rows)");
-  Reader rd{packed, n_floats};
+  PackReader rd{packed, n_floats};
   const int fin = 4 * M + 1;
   kin = round_up(fin, 32);
   const float* Wi = rd.take((size_t)fin * D);
   const float* bi = rd.take(D);
   if (!rd.ok) return jq_fail(JQ_EINVAL, "parameter pack too short (Psiformer input)");
+  if ((options & JQ_OPT_NO_INPUT_BIAS) && !all_zero(bi, D))
+    return jq_fail(JQ_EINVAL, "input_bias=False but the pack's input bias is not zero");
   {
     std::vector<float> wt((size_t)D * kin, 0.f);
     put_t(wt, kin, Wi, D, 0, D, 0, fin, 0);
-    int rc = upload_wt(wt, &Win, &Win_lo);
+    int rc = upload_wt(wt, D, &Win, &Win_lo);
     if (!rc) rc = upload(bi, 4L * D, (void**)&bin);
     if (rc) return rc;
   }
@@ -96,6 +91,11 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
       mb.push_back(rd.take(ly.mdims[j + 1]));
     }
     if (!rd.ok) return jq_fail(JQ_EINVAL, "parameter pack too short (Psiformer layer)");
+    if ((options & JQ_OPT_NO_ATTN_BIAS) &&
+        !(all_zero(bq, D) && all_zero(bk, D) && all_zero(bv, D) && all_zero(bo, D)))
+      return jq_fail(JQ_EINVAL, "with_bias=False but the pack's attention biases are not zero");
+    for (size_t j = 1; j + 1 < ly.mdims.size(); ++j)
+      if (ly.mdims[j] < 1) return jq_fail(JQ_EINVAL, "bad MLP hidden width");
     int rc = upload(l1s, 4L * D, (void**)&ly.ln1_s);
     if (!rc) rc = upload(l1b, 4L * D, (void**)&ly.ln1_b);
     if (!rc) rc = upload(l2s, 4L * D, (void**)&ly.ln2_s);
@@ -109,11 +109,11 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
       bqkv[D + n] = bk[n];
       bqkv[2 * D + n] = bv[n];
     }
-    if (!rc) rc = upload_wt(wqkv, &ly.Wqkv, &ly.Wqkv_lo);
+    if (!rc) rc = upload_wt(wqkv, 3 * D, &ly.Wqkv, &ly.Wqkv_lo);
     if (!rc) rc = upload(bqkv.data(), 12L * D, (void**)&ly.bqkv);
     std::vector<float> wot((size_t)D * D, 0.f);
     put_t(wot, D, wo, D, 0, D, 0, D, 0);
-    if (!rc) rc = upload_wt(wot, &ly.Wo, &ly.Wo_lo);
+    if (!rc) rc = upload_wt(wot, D, &ly.Wo, &ly.Wo_lo);
     if (!rc) rc = upload(bo, 4L * D, (void**)&ly.bo);
     ly.Wm.resize(mw.size());
     ly.Wm_lo.resize(mw.size());
@@ -123,40 +123,22 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
       const int ka = round_up(a, 32);
       std::vector<float> wt((size_t)b * ka, 0.f);
       put_t(wt, ka, mw[j], b, 0, b, 0, a, 0);
-      rc = upload_wt(wt, &ly.Wm[j], &ly.Wm_lo[j]);
+      rc = upload_wt(wt, b, &ly.Wm[j], &ly.Wm_lo[j]);
       if (!rc) rc = upload(mb[j], 4L * b, (void**)&ly.bm[j]);
     }
     if (rc) return rc;
   }
   // outputs (orbital projection, envelope), then the Jastrow
   {
-    korb = round_up(D, 32);
-    const int ncol = K * N;
-    for (int c = 0; c < nch; ++c) {
-      const float* W = rd.take((size_t)D * ncol);
-      if (!W) return jq_fail(JQ_EINVAL, "parameter pack too short (orbital)");
-      std::vector<float> wt((size_t)ncol * korb, 0.f);
-      put_t(wt, korb, W, ncol, 0, ncol, 0, D, 0);
-      int rc = upload_wt(wt, &Worb[c], &Worb_lo[c]);
-      if (rc) return rc;
-    }
-    const size_t ne = (size_t)N * M * K;
-    for (int c = 0; c < nch; ++c) {
-      const float* pp = rd.take(ne);
-      const float* sg = rd.take(ne);
-      if (!pp || !sg) return jq_fail(JQ_EINVAL, "parameter pack too short (envelope)");
-      const std::vector<float> pt = envelope_device_layout(pp, N, M, K);
-      const std::vector<float> st = envelope_device_layout(sg, N, M, K);
-      int rc = upload(pt.data(), ne * 4, (void**)&pi[c]);
-      if (!rc) rc = upload(st.data(), ne * 4, (void**)&sigma[c]);
-      if (rc) return rc;
-    }
+    int rc = read_outputs(rd, D);
+    if (rc) return rc;
   }
   const float* jas_p = rd.take(2);
   if (!jas_p) return jq_fail(JQ_EINVAL, "parameter pack too short (Jastrow)");
   jas[0] = jas_p[0];
   jas[1] = jas_p[1];
   if (rd.pos != n_floats) return jq_fail(JQ_EINVAL, "parameter pack size mismatch");
+  n_params = (long)n_floats;
 
   // ---- workspace ----
   int hmax = D;
@@ -186,7 +168,7 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
     return (size_t)std::max((long)chunk * per_lap, (long)chunk_fwd * per_fwd);
   };
   const long uL = (long)N * Rp, uF = N;
-  P[0] = alloc(4 * nmax(uL * D, uF * D));                                       // x
+  P[0] = alloc(4 * nmax(uL * hmax, uF * hmax));                                 // x (swaps with T1 post-LN)
   P[1] = alloc(4 * nmax(uL * hmax, uF * hmax));                                 // T1
   P[2] = alloc(4 * nmax(uL * std::max(hmax, kin), uF * std::max(hmax, kin)));   // T2 / X0
   P[3] = alloc(4 * nmax(3 * uL * D, 3 * uF * D));                               // QKV (head-split)
@@ -198,7 +180,7 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
   for (int i = 0; i < 6; ++i)
     if (!P[i]) return jq_fail(JQ_ENOMEM, "workspace allocation failed");
   if (!Phi || !env || !det) return jq_fail(JQ_ENOMEM, "workspace allocation failed");
-  return 0;
+  return alloc_mh_scratch();
 }
 
 int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t s) {
@@ -224,7 +206,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
   float* A2 = A1 + (long)Bc * H * N * RP * Np32;
 
   auto gemm = [&](const float* X, long ldx, void* Wt, void* Wlo, long wplane, int K_, float* Y, long ldy, int ncols,
-                  const float* bias, int act, const float* Res) {
+                  const float* bias, int act, const float* Res, int zpad = 0) {
     GemmArgs g{};
     g.X = X;
     g.ldx = ldx;
@@ -247,18 +229,23 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     g.n_up = n_up;
     g.n_dn = n_dn;
     g.wmaj = 1;
+    g.zpad = zpad;
     ProfScope ps(lap ? PROF_GEMM_LAYER_LAP : -1, s);
     return launch_gemm(g, prec, false, s);
   };
 
-  int rc = launch_psi_features(fd, elec, atoms, T2, kin, s);
+  int rc = launch_psi_features(fd, elec, atoms, T2, kin, (options & JQ_OPT_NO_RESCALE) ? 0 : 1, s);
   rc |= gemm(T2, kin, Win, Win_lo, (long)D * kin, kin, x, D, D, bin, ACT_NONE, nullptr);
   for (int l = 0; l < L && !rc; ++l) {
     const PsiLayer& ly = pl[l];
-    rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln1_s, ly.ln1_b, 1e-5f, s);
+    const float* qin = x;  // attention input: LN1(x) pre-LN, x otherwise
+    if (ln_mode == JQ_LN_PRE) {
+      rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln1_s, ly.ln1_b, 1e-5f, s);
+      qin = T1;
+    }
     {
       GemmArgs g{};
-      g.X = T1;
+      g.X = qin;
       g.ldx = D;
       g.Wt = ly.Wqkv;
       g.Wt_lo = ly.Wqkv_lo;
@@ -365,9 +352,16 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     }
     // x += Wo O + bo
     rc |= gemm(Om, DO, ly.Wo, ly.Wo_lo, (long)D * D, D, x, D, D, ly.bo, ACT_NONE, x);
-    // MLP on LN2(x), residual into x
-    rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln2_s, ly.ln2_b, 1e-5f, s);
-    float* cur = T1;
+    if (ln_mode == JQ_LN_POST) {  // x = LN1(x + attention)
+      rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln1_s, ly.ln1_b, 1e-5f, s);
+      std::swap(x, T1);
+    }
+    // MLP on LN2(x) (pre-LN) or x, residual into x
+    float* cur = x;
+    if (ln_mode == JQ_LN_PRE) {
+      rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln2_s, ly.ln2_b, 1e-5f, s);
+      cur = T1;
+    }
     int curw = D;
     const int nm = (int)ly.Wm.size();
     for (int j = 0; j < nm && !rc; ++j) {
@@ -375,10 +369,16 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       const int ka = round_up(curw, 32);
       const bool last = j == nm - 1;
       float* dst = last ? x : (cur == T1 ? T2 : T1);
-      rc |= gemm(cur, last ? ka : ka, ly.Wm[j], ly.Wm_lo[j], (long)outw * ka, ka, dst, last ? D : round_up(outw, 32),
-                 outw, ly.bm[j], ACT_TANH, last ? x : nullptr);
+      // hidden outputs: padding columns up to the next K multiple are zeroed
+      // for the following GEMM
+      rc |= gemm(cur, ka, ly.Wm[j], ly.Wm_lo[j], (long)outw * ka, ka, dst, last ? D : round_up(outw, 32), outw,
+                 ly.bm[j], ACT_TANH, last ? x : nullptr, last ? 0 : round_up(outw, 32));
       cur = dst;
       curw = outw;
+    }
+    if (ln_mode == JQ_LN_POST) {  // x = LN2(x + MLP)
+      rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln2_s, ly.ln2_b, 1e-5f, s);
+      std::swap(x, T1);
     }
   }
   if (rc) return jq_fail(JQ_EINVAL, "Psiformer launch rejected");
@@ -399,7 +399,16 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     go.K = korb;
     go.ncols = K * N;
     go.R = 1;
+    go.lrp = lap ? RP : 0;
     go.act = ACT_NONE;
+    go.bias = borb[c];
+    if (lap && borb[c]) {  // orbital bias on the value rows only: unit-tiled launch
+      go.R = RP;
+      go.wmaj = 1;
+      go.Bc = Bc;
+      go.n_up = n_up;
+      go.n_dn = n_dn;
+    }
     go.nbatch = Bc;
     go.sxb = (long)N * RP * D;
     go.syb = (long)N * RP * ldphi;
@@ -409,6 +418,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
   const float* pis[2] = {pi[0], pi[1]};
   const float* sgs[2] = {sigma[0], sigma[1]};
   launch_envelope(fd, elec, atoms, pis, sgs, d.envelope, env, s);
+  if (skip_det) return 0;
   ProfScope ps(lap ? PROF_DET_LAP : -1, s);
   if (launch_det(fd, Phi, ldphi, env, det, s)) return jq_fail(JQ_EINVAL, "determinant launch rejected");
   return 0;

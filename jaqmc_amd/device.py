@@ -27,12 +27,17 @@ def _stream():
 class DeviceModel:
     """A FermiNet or Psiformer wavefunction on one GPU.
 
-    precision: "fp32" (FP32 MFMA layers), "bf16" (BF16 MFMA operands and
-    storage, FP32 accumulation) or "bf16x3" (split-BF16 MFMA, FP32 storage).
-    Determinants, E_L assembly and reductions are FP64 in every mode.
+    precision: arithmetic of the layer / tangent GEMMs.  "fp32" (FP32 MFMA);
+    "bf16" (one BF16 term per operand); "bf16x3" (two BF16 terms, three
+    products, ~2^-16); "bf16x6" (three BF16 terms, six products, FP32-level);
+    "fp16x3" (two FP16 terms, three products, ~2^-22, power-of-two row and
+    column scaling for FP16's range; value-only launches use bf16x6).  Every
+    mode accumulates in FP32 and keeps activations FP32 in HBM; determinants,
+    E_L assembly and reductions are FP64 in every mode.
     """
 
-    def __init__(self, spec: ModelSpec, atoms, charges, params, precision="fp32", chunk_walkers=0):
+    def __init__(self, spec: ModelSpec, atoms, charges, params, precision="fp32", chunk_walkers=0,
+                 with_gradients=False):
         lib = _lib.load()
         if not torch.cuda.is_available():
             raise _lib.JqError("DeviceModel needs a ROCm GPU (torch.cuda.is_available() is False)")
@@ -61,6 +66,8 @@ class DeviceModel:
             d.jastrow = 1 if spec.jastrow else 0
         d.envelope = _lib.ENVELOPE[spec.envelope]
         d.precision = _lib.PRECISION[precision]
+        d.layer_norm_mode = _lib.LAYER_NORM[spec.layer_norm_mode] if spec.kind == "psiformer" else 0
+        d.options = spec.options() | (_lib.OPT_WITH_GRADIENTS if with_gradients else 0)
         d.chunk_walkers = int(chunk_walkers)
         d.atoms = self.atoms.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         d.charges = self.charges.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
@@ -116,6 +123,49 @@ class DeviceModel:
             out["grad"] = grad
         return out
 
+    @property
+    def n_params(self):
+        return int(self._lib.jq_model_param_count(self._h))
+
+    def logpsi_vjp(self, elec, cot):
+        """sum_w cot[w] d log|psi(x_w)| / d theta as a float32 vector in pack
+        order (FermiNet models created with with_gradients=True)."""
+        B = self._check_elec(elec)
+        if not (cot.is_cuda and cot.dtype == torch.float64 and cot.numel() == B and cot.is_contiguous()):
+            raise ValueError("cot must be a contiguous float64 CUDA tensor of B entries")
+        g = torch.empty(self.n_params, dtype=torch.float32, device=elec.device)
+        _lib.check(self._lib.jq_logpsi_vjp(self._h, _ptr(elec), B, _ptr(cot), _ptr(g), _stream()), "jq_logpsi_vjp")
+        return g
+
+    def orbitals(self, elec):
+        """(B, K, N, N) orbital matrices [walker, det, electron, orbital] (FP32)."""
+        B = self._check_elec(elec)
+        K = self.spec.ndets
+        out = torch.empty(B, K, self.N, self.N, dtype=torch.float32, device=elec.device)
+        _lib.check(self._lib.jq_orbitals(self._h, _ptr(elec), B, _ptr(out), _stream()), "jq_orbitals")
+        return out
+
+    def sample(self, elec, state, n_outer, steps, adapt_frequency=100, pmove_range=(0.5, 0.55), seed=0, ctr=0,
+               n_accept=None, logp=None, pmove_out=None):
+        """n_outer device-side MCMCSampler.step calls (moves + width adaptation,
+        no host synchronisation).  ``state`` is the float64 CUDA tensor
+        {stddev, counter, accepted total, pmoves[adapt_frequency]}; returns
+        (logp, n_accept)."""
+        B = self._check_elec(elec)
+        if not (state.is_cuda and state.dtype == torch.float64 and state.numel() == 3 + adapt_frequency):
+            raise ValueError("state must be a float64 CUDA tensor of 3 + adapt_frequency entries")
+        if n_accept is None:
+            n_accept = torch.zeros(B, dtype=torch.int32, device=elec.device)
+        if logp is None:
+            logp = torch.empty(B, dtype=torch.float32, device=elec.device)
+        if pmove_out is not None and not (pmove_out.dtype == torch.float64 and pmove_out.numel() >= n_outer):
+            raise ValueError("pmove_out must be a float64 CUDA tensor of n_outer entries")
+        _lib.check(self._lib.jq_sample(self._h, _ptr(elec), _ptr(logp), B, int(n_outer), int(steps), _ptr(state),
+                                       int(adapt_frequency), float(pmove_range[0]), float(pmove_range[1]),
+                                       ctypes.c_uint64(seed), ctypes.c_uint64(ctr), _ptr(n_accept), _ptr(pmove_out),
+                                       _stream()), "jq_sample")
+        return logp, n_accept
+
     def mh_steps(self, elec, logp, n_steps, width, normals=None, uniforms=None, seed=0, ctr=0, n_accept=None):
         """In-place Metropolis moves; returns the int32 accept counts (B,)."""
         B = self._check_elec(elec)
@@ -140,6 +190,26 @@ def reduce_mean_var(x):
     out = torch.empty(3, dtype=torch.float64, device=x.device)
     _lib.check(lib.jq_reduce_mean_var(_ptr(x), x.numel(), _ptr(out), _stream()), "jq_reduce_mean_var")
     return out
+
+
+CLIP_METHODS = {"mad": 0, "iqr": 1, "none": 2}
+
+
+def loss_clip(x, method="mad", scale=5.0, workspace=None):
+    """LossAndGrad's clipping on the GPU (FP64): returns (clipped, weights,
+    stats) with weights = 2 (clipped - mean(clipped)) / n and stats =
+    {nanmean(x), nanmean(clipped), lo, hi, n}."""
+    lib = _lib.load()
+    B = x.numel()
+    nb = int(lib.jq_loss_clip_workspace_bytes(B))
+    if workspace is None or workspace.numel() < nb:
+        workspace = torch.empty(nb, dtype=torch.uint8, device=x.device)
+    clipped = torch.empty(B, dtype=torch.float64, device=x.device)
+    w = torch.empty(B, dtype=torch.float64, device=x.device)
+    stats = torch.empty(5, dtype=torch.float64, device=x.device)
+    _lib.check(lib.jq_loss_clip(_ptr(x), B, CLIP_METHODS[method], float(scale), _ptr(clipped), _ptr(w), _ptr(stats),
+                                _ptr(workspace), nb, _stream()), "jq_loss_clip")
+    return clipped, w, stats
 
 
 def reduce_accepts(n_accept):

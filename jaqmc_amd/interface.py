@@ -15,7 +15,10 @@ protocols drives the HIP kernels unchanged:
   estimator/kinetic/euclidean.py:29-135, estimator/total_energy.py:16-70,
   app/molecule/hamiltonian.py:9-22);
 * :func:`evaluation_step` -- EvaluationWorkStage.compute_step
-  (workflow/stage/evaluation.py:179-199).
+  (workflow/stage/evaluation.py:179-199);
+* :class:`LossAndGrad` -- the VMC loss and parameter gradient
+  (estimator/loss_grad.py:24-128): per-walker value-and-grad, clipping,
+  and the baseline-subtracted mean, here contracted on the device.
 
 Differences forced by the device boundary: the wavefunction methods are
 *batched* (the reference's are single-walker and run under ``jax.vmap``;
@@ -32,8 +35,8 @@ from typing import Any, Dict, NamedTuple, Optional
 import numpy as np
 import torch
 
-from .device import DeviceModel, reduce_accepts, reduce_mean_var
-from .params import ModelSpec, init_params
+from .device import DeviceModel, loss_clip, reduce_accepts, reduce_mean_var
+from .params import ModelSpec, init_params, unpack
 
 
 @dataclasses.dataclass
@@ -54,9 +57,10 @@ class _MoleculeWavefunction:
 
     spec: ModelSpec
 
-    def __init__(self, precision: str = "fp32", chunk_walkers: int = 0):
+    def __init__(self, precision: str = "fp32", chunk_walkers: int = 0, with_gradients: bool = False):
         self.precision = precision
         self.chunk_walkers = chunk_walkers
+        self.with_gradients = with_gradients
         self._models: Dict[int, DeviceModel] = {}
 
     def init_params(self, data: MoleculeData, rngs: int = 0, bias_scale: float = 0.0):
@@ -68,7 +72,7 @@ class _MoleculeWavefunction:
         m = self._models.get(key)
         if m is None:
             m = DeviceModel(self.spec, data.atoms, data.charges, params, precision=self.precision,
-                            chunk_walkers=self.chunk_walkers)
+                            chunk_walkers=self.chunk_walkers, with_gradients=self.with_gradients)
             self._models = {key: m}  # one resident parameter set at a time
         return m
 
@@ -83,17 +87,23 @@ class _MoleculeWavefunction:
         out = self.evaluate(params, data)
         return out["sign_logpsi"], out["logpsi"]
 
+    def orbitals(self, params, data: MoleculeData):
+        """(B, K, N, N) orbital matrices, envelope applied
+        (app/molecule/wavefunction/ferminet.py:128-138)."""
+        return self.model(params, data).orbitals(data.electrons)
+
 
 class FermiNetWavefunction(_MoleculeWavefunction):
     """FermiNet (app/molecule/wavefunction/ferminet.py:21-74); full determinants,
     abs_isotropic envelope and spin-split orbitals by default."""
 
     def __init__(self, nspins, n_atoms, ndets=16, hidden_dims_single=(256,) * 4, hidden_dims_double=(32,) * 4,
-                 envelope="abs_isotropic", precision="fp32", chunk_walkers=0):
-        super().__init__(precision, chunk_walkers)
+                 envelope="abs_isotropic", precision="fp32", chunk_walkers=0, orbitals_spin_split=True,
+                 with_gradients=False):
+        super().__init__(precision, chunk_walkers, with_gradients)
         self.spec = ModelSpec("ferminet", tuple(nspins), n_atoms, ndets=ndets,
                               hidden_single=list(hidden_dims_single), hidden_double=list(hidden_dims_double),
-                              envelope=envelope)
+                              envelope=envelope, orbitals_spin_split=orbitals_spin_split)
 
 
 class PsiformerWavefunction(_MoleculeWavefunction):
@@ -101,11 +111,14 @@ class PsiformerWavefunction(_MoleculeWavefunction):
     SimpleEE Jastrow, rescaled features."""
 
     def __init__(self, nspins, n_atoms, ndets=16, num_layers=4, num_heads=4, heads_dim=64, mlp_hidden_dims=(256,),
-                 jastrow=True, envelope="abs_isotropic", precision="fp32", chunk_walkers=0):
+                 jastrow=True, envelope="abs_isotropic", precision="fp32", chunk_walkers=0, layer_norm_mode="pre",
+                 with_bias=True, input_bias=True, bias_orbitals=False, rescale=True, orbitals_spin_split=True):
         super().__init__(precision, chunk_walkers)
         self.spec = ModelSpec("psiformer", tuple(nspins), n_atoms, ndets=ndets, num_layers=num_layers,
                               num_heads=num_heads, heads_dim=heads_dim, mlp_hidden=list(mlp_hidden_dims),
-                              jastrow=jastrow, envelope=envelope)
+                              jastrow=jastrow, envelope=envelope, layer_norm_mode=layer_norm_mode,
+                              with_bias=with_bias, input_bias=input_bias, bias_orbitals=bias_orbitals,
+                              rescale=rescale, orbitals_spin_split=orbitals_spin_split)
 
 
 class BatchLogProb:
@@ -229,6 +242,43 @@ class EstimatorPipeline:
                 s, _ = est.evaluate_batch_walkers(params, batched_data, walker_stats, None, rngs)
             walker_stats.update(s)
         return walker_stats, mean_reduce(walker_stats)
+
+
+class LossAndGrad:
+    """VMC loss and parameter gradient (estimator/loss_grad.py:24-128).
+
+    ``evaluate_batch_walkers`` keeps the per-walker loss (``loss_key`` of the
+    previous estimators) and log|psi|; ``reduce`` clips the losses on the
+    device (jq_loss_clip: "mad" / "iqr" / "none", FP64) and forms the walker
+    weights 2 (c_w - mean c) / n; ``finalize_stats`` contracts the weights with
+    the per-walker parameter gradients in one device pass (jq_logpsi_vjp),
+    which equals the reference's 2 (mean(grad c) - mean(c) mean(grad)).  The
+    gradient is returned flat (pack order) and as the parameter tree.
+    """
+
+    def __init__(self, wf: _MoleculeWavefunction, loss_key="total_energy", clip_method="mad", clip_scale=5.0):
+        if not wf.with_gradients:
+            raise ValueError("LossAndGrad needs a wavefunction created with with_gradients=True")
+        self.wf, self.loss_key = wf, loss_key
+        self.clip_method, self.clip_scale = clip_method, clip_scale
+        self._ctx = None
+
+    def evaluate_batch_walkers(self, params, batched_data: MoleculeData, prev_walker_stats, state=None, rngs=None):
+        loss = prev_walker_stats[self.loss_key].double().contiguous()
+        self._ctx = (params, batched_data)
+        return {"logpsi": self.wf.logpsi(params, batched_data), "loss": loss}, state
+
+    def reduce(self, walker_stats):
+        clipped, w, st = loss_clip(walker_stats["loss"], self.clip_method, self.clip_scale)
+        return {"weights": w, "clipped_loss": clipped, "stats": st}
+
+    def finalize_stats(self, batch_stats, state=None):
+        params, data = self._ctx
+        m = self.wf.model(params, data)
+        g = m.logpsi_vjp(data.electrons, batch_stats["weights"]).cpu().numpy()
+        st = batch_stats["stats"].cpu().numpy()
+        return {"loss": float(st[0]), "clipped_loss": float(st[1]), "grads_flat": g,
+                "grads": unpack(self.wf.spec, g, params)}
 
 
 def evaluation_step(wf, sampler: MCMCSampler, pipeline: EstimatorPipeline, params, data: MoleculeData,

@@ -15,6 +15,7 @@ PEAKS = {  # MI355X published dense peaks
     "fp32": 157.3e12,
     "bf16": 2.5e15,
     "bf16x3": 2.5e15,
+    "fp16": 2.5e15,
     "fp64": 78.6e12,
     "hbm": 8.0e12,
 }
@@ -132,4 +133,6 @@ def layer_bytes(spec: ModelSpec, n_atoms: int):
             else psiformer_layer_bytes(spec, n_atoms))
 
 
-MFMA_PER_PRODUCT = {"fp32": 1, "bf16": 1, "bf16x3": 3, "bf16x6": 6}
+MFMA_PER_PRODUCT = {"fp32": 1, "bf16": 1, "bf16x3": 3, "bf16x6": 6, "fp16x3": 3}
+# dense MFMA peak of each mode's operand type
+MODE_PEAK = {"fp32": "fp32", "bf16": "bf16", "bf16x3": "bf16", "bf16x6": "bf16", "fp16x3": "fp16"}

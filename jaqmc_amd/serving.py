@@ -17,7 +17,14 @@ Scheduling is iteration-level continuous batching:
   (different electron counts, different kernels) run concurrently on the
   device;
 * a job leaves the batch at its last iteration; its slot and memory are
-  reused by the next queued job on the following tick.
+  reused by the next queued job on the following tick;
+* device state is built off the scheduler thread: an admitted job's runner
+  (weight upload, workspace, initial log|psi|) is constructed by a builder
+  thread while the scheduler keeps enqueueing the running jobs' iterations,
+  and the job joins the batch on the first tick after its runner is ready;
+  teardown waits only for the job's own stream (jq_model_destroy synchronises
+  nothing device-wide and returns memory to the stream-ordered pool, where the
+  next job's workspace is carved from it).
 
 Results do not depend on the mix: each job's Metropolis randomness is Philox
 keyed by (job seed, iteration, walker), its kernels read only its own buffers,
@@ -28,6 +35,7 @@ bit-identical energies.
 from __future__ import annotations
 
 import collections
+import concurrent.futures as cf
 import dataclasses
 import time
 from typing import Callable, Deque, Dict, List, Optional
@@ -116,8 +124,12 @@ class Server:
     """
 
     def __init__(self, runner_factory: Callable, mem_budget: int, max_active: int = 8,
-                 footprint: Callable = None, clock: Callable = time.perf_counter):
+                 footprint: Callable = None, clock: Callable = time.perf_counter, async_build: bool = True,
+                 builders: int = 2):
         self.factory = runner_factory
+        self.async_build = async_build
+        self._pool = cf.ThreadPoolExecutor(builders) if async_build else None
+        self.starting: List[dict] = []
         self.mem_budget = mem_budget
         self.max_active = max_active
         self.footprint = footprint or (lambda job: footprint_bytes(job.spec, job.atoms.shape[0], job.walkers))
@@ -136,17 +148,33 @@ class Server:
 
     def _admit(self):
         # FIFO admission; a large job at the head waits for memory rather than
-        # being overtaken (no starvation).
-        while self.queue and len(self.active) < self.max_active:
+        # being overtaken (no starvation).  Memory and the slot are reserved at
+        # admission; the runner is built by a builder thread (async_build) or
+        # here.
+        while self.queue and len(self.active) + len(self.starting) < self.max_active:
             job = self.queue[0]
             need = self.footprint(job)
             if self.mem_used + need > self.mem_budget:
                 break
             self.queue.popleft()
-            runner = self.factory(job)
             self.mem_used += need
-            self.active.append(dict(job=job, runner=runner, next=0, need=need, admitted=self.clock()))
-            self.admit_log.append((job.job_id, len(self.active), self.mem_used))
+            ent = dict(job=job, next=0, need=need, admitted=self.clock())
+            if self.async_build:
+                ent["future"] = self._pool.submit(self.factory, job)
+                self.starting.append(ent)
+            else:
+                ent["runner"] = self.factory(job)
+                self.active.append(ent)
+            self.admit_log.append((job.job_id, len(self.active) + len(self.starting), self.mem_used))
+        # runners that finished building join the batch
+        still = []
+        for ent in self.starting:
+            if ent["future"].done():
+                ent["runner"] = ent.pop("future").result()
+                self.active.append(ent)
+            else:
+                still.append(ent)
+        self.starting = still
         self.peak_active = max(self.peak_active, len(self.active))
 
     def tick(self):
@@ -169,7 +197,7 @@ class Server:
             else:
                 still.append(a)
         self.active = still
-        return bool(self.queue or self.active)
+        return bool(self.queue or self.active or self.starting)
 
     def run(self, arrivals: Optional[List[Job]] = None, idle_sleep: float = 0.0005):
         """Serve until every submitted job (and every timed arrival) is done.
@@ -182,13 +210,17 @@ class Server:
             busy = self.tick()
             if not busy and not pending:
                 break
-            if not self.active and pending:
+            if not self.active and not self.starting and pending:
                 wait = pending[0].arrival - self.clock()
                 if wait > 0:
                     time.sleep(min(wait, 0.01))
             elif idle_sleep:
                 time.sleep(idle_sleep)
         return self.results
+
+    def close(self):
+        if self._pool is not None:
+            self._pool.shutdown(wait=True)
 
 
 class GpuRunner:

@@ -10,9 +10,21 @@ imports it and fails loudly when its HIP library is missing.
 
 How each function is pinned (checks that a plausible mistake would fail):
 
-* ``lap.Lap`` rules and the whole ``wavefunction.logpsi`` Laplacian: against a
-  brute-force float64 autograd Hessian trace (torch double backward) on small
-  systems, and against central finite differences -- tests/test_oracle.py.
+* ``lap.Lap`` rules and the whole ``wavefunction.logpsi`` pipeline (FermiNet
+  and Psiformer, every LayerNorm placement and module option): value, gradient
+  and Laplacian against an independent float64 torch evaluation built from
+  torch library operations (layer_norm, softmax, linalg.slogdet) with autograd
+  gradients and the autograd Hessian trace (double backward) --
+  tests/test_oracle_pins.py; and against central finite differences --
+  tests/test_oracle.py.
+* ``layer_norm``: zero mean and mean square v/(v+eps) per row, a variance-eps
+  row (eps inside the rsqrt), a constant row (gives the bias), torch's
+  layer_norm.
+* ``mha``: uniform weights when scores vanish, one electron, a hand-computed
+  two-electron softmax (pins the 1/sqrt(dh) placement), torch's
+  scaled_dot_product_attention.
+* ``aggregate``: hand values of an asymmetric two-electron stream (mean over
+  the first index), two channels and polarized.
 * kinetic machinery: closed forms -- hydrogen-like logpsi = -r gives
   E_L = -1/2 exactly; harmonic logpsi = -c sum x^2 gives KE = c*n*D - 2c^2 sum x^2
   (reference tests: tests/hydrogen/atom_test.py:31-53,
@@ -27,10 +39,10 @@ How each function is pinned (checks that a plausible mistake would fail):
   Gaussian |psi|^2.
 * reductions: float64 two-pass vs numpy.
 
-Parity unpinned: flax's LayerNorm / MultiHeadDotProductAttention internals
-(variance formula, scaling placement) are restated from flax's published
-definitions; no reference fixture pins them.  The JAX threefry PRNG stream is
-not reproduced: parity runs feed explicit normals and uniforms to both sides.
+Parity unpinned: only what no check here can reach -- the JAX threefry PRNG
+stream (parity runs feed explicit normals and uniforms to both sides instead)
+and flax's float32 summation order inside LayerNorm's variance (the oracle
+runs in float64, where it does not matter).
 """
 
 from . import hamiltonian, lap, wavefunction  # noqa: F401

@@ -152,8 +152,12 @@ def mha(x: Lap, p, num_heads):
     return o.matmul(p["wo"], p["bo"])
 
 
-def psiformer_backbone(params, feats, nspins, num_heads):
-    """PsiformerBackbone with pre-LN layers (wavefunction/backbone/psiformer.py:64-187)."""
+def psiformer_backbone(params, feats, nspins, num_heads, ln_mode="pre"):
+    """PsiformerBackbone (wavefunction/backbone/psiformer.py:64-187).
+
+    PsiformerLayer (psiformer.py:70-98): pre-LN  x += MHA(LN1(x)); x += MLP(LN2(x));
+    post-LN  x = LN1(x + MHA(x)); x = LN2(x + MLP(x));  null: no LayerNorm.
+    """
     h = feats["ae_features"]
     B, N = h.shape[:2]
     n_up, n_dn = nspins
@@ -161,12 +165,16 @@ def psiformer_backbone(params, feats, nspins, num_heads):
     h = concat([h, Lap.const(spins, B, h.T, h.dtype)], -1)
     x = h.matmul(params["input"][0], params["input"][1])
     for lp in params["layers"]:
-        xin = layer_norm(x, lp["ln1_scale"], lp["ln1_bias"])
+        xin = layer_norm(x, lp["ln1_scale"], lp["ln1_bias"]) if ln_mode == "pre" else x
         x = x + mha(xin, lp, num_heads)
-        m = layer_norm(x, lp["ln2_scale"], lp["ln2_bias"])
+        if ln_mode == "post":
+            x = layer_norm(x, lp["ln1_scale"], lp["ln1_bias"])
+        m = layer_norm(x, lp["ln2_scale"], lp["ln2_bias"]) if ln_mode == "pre" else x
         for (W, b) in lp["mlp"]:
             m = m.matmul(W, b).tanh()
         x = x + m
+        if ln_mode == "post":
+            x = layer_norm(x, lp["ln2_scale"], lp["ln2_bias"])
     return x
 
 
@@ -188,12 +196,16 @@ def jastrow(r_ee: Lap, nspins, alpha_par, alpha_anti):
 # orbitals, envelope, determinants
 # --------------------------------------------------------------------------
 def orbitals(params, h: Lap, nspins, ndets):
-    """OrbitalProjection with spin split (wavefunction/output/orbital.py:60-78).
+    """OrbitalProjection (wavefunction/output/orbital.py:60-78): one DenseGeneral
+    per occupied spin channel (SplitChannelDense) when the tree holds one
+    kernel per channel, else one shared DenseGeneral; optional bias.
 
     Returns (B, K, N, N): [det, electron, orbital].
     """
     N = sum(nspins)
     chans = spin_channels(nspins)
+    if len(params["orbital"]) == 1:
+        chans = [(0, N)]
     blocks = []
     for c, (a, b) in enumerate(chans):
         W = params["orbital"][c]
@@ -213,6 +225,8 @@ def envelope(params, r_ae: Lap, nspins, ndets, kind):
     if kind == "null":
         return None
     chans = spin_channels(nspins)
+    if len(params["envelope"]) == 1:  # shared envelope (orbitals_spin_split=False)
+        chans = [(0, N)]
     blocks = []
     for c, (a, b) in enumerate(chans):
         pi, sigma = params["envelope"][c]  # (N_orb, M, K)
@@ -256,8 +270,8 @@ def logpsi(model, params, x, atoms, with_laplacian=True):
         feats = molecule_features(xl, atoms, rescale=False)
         h = ferminet_backbone(params, feats, nspins)
     else:
-        feats = molecule_features(xl, atoms, rescale=True)
-        h = psiformer_backbone(params, feats, nspins, model["num_heads"])
+        feats = molecule_features(xl, atoms, rescale=model.get("rescale", True))
+        h = psiformer_backbone(params, feats, nspins, model["num_heads"], model.get("layer_norm_mode", "pre"))
     phi = orbitals(params, h, nspins, K)
     env = envelope(params, feats["r_ae"], nspins, K, model.get("envelope", "abs_isotropic"))
     A = phi * env if env is not None else phi
@@ -278,8 +292,8 @@ def orbital_matrices(model, params, x, atoms):
         feats = molecule_features(xl, atoms, rescale=False)
         h = ferminet_backbone(params, feats, nspins)
     else:
-        feats = molecule_features(xl, atoms, rescale=True)
-        h = psiformer_backbone(params, feats, nspins, model["num_heads"])
+        feats = molecule_features(xl, atoms, rescale=model.get("rescale", True))
+        h = psiformer_backbone(params, feats, nspins, model["num_heads"], model.get("layer_norm_mode", "pre"))
     phi = orbitals(params, h, nspins, K)
     env = envelope(params, feats["r_ae"], nspins, K, model.get("envelope", "abs_isotropic"))
     return (phi * env).v if env is not None else phi.v

@@ -1,27 +1,36 @@
 """GPU parity of the HIP path against the CPU oracle, through the C ABI.
 
 Tolerances (FP32 path vs the FP64 oracle on FP32 parameters and FP32 walker
-positions):
+positions; u = 2^-24, kappa = the walker's largest 2-norm condition number of
+its orbital matrices, from the oracle; bounds in tests/parity_util.py):
   log|psi|  : |d| <= 1e-5 |log psi| + 1e-4 (forward path vs the oracle);
               the Laplacian-mode value vs the forward path also allows the
-              first-order rounding bound of log|det A|, 4 u kappa(A) with
-              u = 2^-24 and kappa the largest 2-norm condition number of the
-              walker's orbital matrices (the two paths round differently)
-  E_L       : median relative error <= 2e-5, max <= 2e-3 (relative to max(1, |E_L|));
-              walkers near a node amplify FP32 rounding of the orbital tangents
+              first-order rounding bound of log|det A|, 4 u kappa(A) (the two
+              paths round differently)
+  E_L       : median relative error <= 2e-5 and, per walker, relative error
+              <= 1e-5 + 2 e kappa, e = 64 u the relative error of the orbital
+              entries (first-order amplification by the inverse); the fraction
+              of walkers within 1e-5 is printed
   grad      : |d| <= 2e-4 max(1, |grad|) per component (median), 5e-3 (max)
-Metropolis decisions with identical proposals and uniforms agree except where
-|ratio - log u| < 1e-4 (the FP32 log-ratio error; the reference's FP32 JAX
-path has the same limit).
+Metropolis decisions with identical proposals and uniforms agree wherever the
+oracle's log ratio is farther from log u than 1e-6 plus twice the measured
+difference between the GPU's and the oracle's log ratio (a decision can only
+flip when the two log ratios straddle log u); the count inside that band is
+printed.
 """
 
 import numpy as np
 import pytest
 
+import os
+
 from jaqmc_amd import params as P
 from jaqmc_amd import systems as S
 from oracle import hamiltonian as ham
 from oracle import wavefunction as owf
+from parity_util import el_report, logpsi_bound
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 pytestmark = pytest.mark.gpu
 
@@ -99,6 +108,17 @@ def _setup(case, B):
         setattr(spec, k, v)
     p = P.init_params(spec, seed=11, bias_scale=0.1, env_jitter=0.2)
     x = S.init_walkers(sysm, B, seed=12)
+    fx = os.path.join(GOLDEN, f"parity_{case}.npz")
+    if case in CASES and os.path.exists(fx):
+        # FP64 oracle references written by tests/golden/make_parity_fixtures.py
+        # from the same seeds (the stored walkers are checked against them)
+        with np.load(fx, allow_pickle=False) as f:
+            ref = {k: f[k] for k in f.files}
+        if ref["x"].shape[0] >= B:
+            np.testing.assert_array_equal(ref["x"][:B], x)
+            ref = {k: v[:B] for k, v in ref.items()}
+            _cache[key] = (sysm, spec, p, x, ref)
+            return _cache[key]
     p64 = P.cast_params(p, np.float64)
     ref = ham.local_energy(spec.as_oracle(), p64, x.astype(np.float64), sysm.atoms.astype(np.float64),
                            sysm.charges.astype(np.float64))
@@ -114,8 +134,8 @@ def _model(sysm, spec, p, prec="fp32", chunk=0):
     return DeviceModel(spec, sysm.atoms, sysm.charges, p, precision=prec, chunk_walkers=chunk)
 
 
-@pytest.mark.parametrize("case,B", [("li", 16), ("n2_small", 16), ("n2", 12), ("h60", 4), ("n2_iso", 8),
-                                    ("psi_small", 8), ("ring42", 2), ("h_atom", 16), ("li_polarized", 16),
+@pytest.mark.parametrize("case,B", [("li", 16), ("n2_small", 16), ("n2", 32), ("h60", 8), ("n2_iso", 8),
+                                    ("psi_small", 8), ("ring42", 8), ("h_atom", 16), ("li_polarized", 16),
                                     ("h17", 6), ("h24", 6)])
 def test_logpsi_and_local_energy_fp32(cuda, case, B):
     import torch
@@ -132,9 +152,9 @@ def test_logpsi_and_local_energy_fp32(cuda, case, B):
     d_lap = np.abs(out["logpsi"].cpu().numpy() - lp)
     assert np.all(d_lap <= 1e-5 * np.abs(lp) + 1e-4 + 4 * 2.0**-24 * ref["kappa"]), (d_lap, ref["kappa"])
     np.testing.assert_allclose(out["e_pot"].cpu().numpy(), ref["e_pot"], rtol=1e-6)
-    rel = np.abs(out["e_loc"].cpu().numpy() - ref["e_loc"]) / np.maximum(1.0, np.abs(ref["e_loc"]))
+    rel, bnd = el_report(f"{case} B={B}", out["e_loc"].cpu().numpy(), ref["e_loc"], ref["kappa"])
     assert np.median(rel) <= 2e-5, rel
-    assert rel.max() <= 2e-3, rel
+    assert np.all(rel <= bnd), (rel, bnd)
     ek = out["e_kin"].cpu().numpy()
     np.testing.assert_allclose(ek + out["e_pot"].cpu().numpy(), out["e_loc"].cpu().numpy(), rtol=1e-12)
     g = out["grad"].cpu().numpy()
@@ -172,17 +192,28 @@ def test_metropolis_decisions_match_oracle(cuda, case):
     acc = dm.mh_steps(xe, logp, steps, width, normals=torch.from_numpy(normals).to(cuda),
                       uniforms=torch.from_numpy(uniforms).to(cuda))
     torch.cuda.synchronize()
-    # oracle, step by step from the GPU's own starting state of each step
+    # oracle, step by step; the band of each decision is 1e-6 plus twice the
+    # measured difference of the GPU's and the oracle's log ratio on the same
+    # FP32 positions (GPU log|psi| from jq_logpsi, which the Metropolis
+    # forward pass shares)
     xo = x.astype(np.float64)
     lpo = 2 * ref["logpsi"]
     p64 = P.cast_params(p, np.float64)
+    at64 = sysm.atoms.astype(np.float64)
     nacc = np.zeros(B, int)
     ambiguous = np.zeros(B, bool)
     for s in range(steps):
-        xo, lpo, a, ratio, logu = ham.mh_step(spec.as_oracle(), p64, xo, lpo, sysm.atoms.astype(np.float64), width,
-                                              normals[s].astype(np.float64), uniforms[s].astype(np.float64))
-        ambiguous |= np.abs(ratio - logu) < 1e-4
+        xp32 = (xo.astype(np.float32) + np.float32(width) * normals[s]).astype(np.float32)
+        lg_x = 2 * dm.logpsi(torch.from_numpy(np.ascontiguousarray(xo.astype(np.float32))).to(cuda))[0].cpu().numpy()
+        lg_p = 2 * dm.logpsi(torch.from_numpy(xp32).to(cuda))[0].cpu().numpy()
+        xo, lpo_new, a, ratio, logu = ham.mh_step(spec.as_oracle(), p64, xo, lpo, at64, width,
+                                                  normals[s].astype(np.float64), uniforms[s].astype(np.float64))
+        delta = np.abs((lg_p - lg_x) - ratio)
+        ambiguous |= np.abs(ratio - logu) < 1e-6 + 2 * delta
+        lpo = lpo_new
         nacc += a
+    print(f"[parity] metropolis {case}: {int(ambiguous.sum())} of {B} walkers with a decision inside the "
+          f"FP32 band, max |ratio error| {delta.max():.2e}", flush=True)
     ok = ~ambiguous
     np.testing.assert_array_equal(acc.cpu().numpy()[ok], nacc[ok])
     np.testing.assert_allclose(xe.cpu().numpy()[ok], xo[ok], atol=1e-5)
@@ -254,21 +285,47 @@ def test_full_size_n2_batch_properties(cuda):
     assert np.median(rel) < 2e-5
 
 
-@pytest.mark.parametrize("prec,lp_tol,med_tol", [("bf16x6", 2e-4, 5e-5), ("bf16x3", 5e-3, 1e-3)])
+@pytest.mark.parametrize("prec,lp_tol,med_tol", [("bf16x6", 2e-4, 5e-5), ("bf16x3", 5e-3, 1e-3),
+                                               ("fp16x3", 5e-4, 5e-5)])
 def test_mixed_precision_modes(cuda, prec, lp_tol, med_tol):
-    """Split-BF16 MFMA modes against the FP64 oracle on N2: bf16x6 (three BF16
-    terms, six products) stays at FP32-level error; bf16x3 (two terms) is ~2^-16."""
+    """Split 16-bit MFMA modes against the FP64 oracle on N2: bf16x6 (three
+    BF16 terms, six products) stays at FP32-level error; bf16x3 (two terms) is
+    ~2^-16; fp16x3 (two FP16 terms, three products) ~2^-22."""
     import torch
 
-    sysm, spec, p, x, ref = _setup("n2", 12)
+    sysm, spec, p, x, ref = _setup("n2", 32)
     dm = _model(sysm, spec, p, prec=prec)
     xe = torch.from_numpy(x).to(cuda)
     lp = dm.logpsi(xe)[0]
     out = dm.local_energy(xe)
     torch.cuda.synchronize()
     assert np.abs(lp.cpu().numpy() - ref["logpsi"]).max() < lp_tol
-    rel = np.abs(out["e_loc"].cpu().numpy() - ref["e_loc"]) / np.maximum(1.0, np.abs(ref["e_loc"]))
+    rel, bnd = el_report(f"n2 B=32", out["e_loc"].cpu().numpy(), ref["e_loc"], ref["kappa"],
+                         prec if prec != "bf16x3" else "fp16x3")
     assert np.median(rel) < med_tol
+    if prec != "bf16x3":
+        assert np.all(rel <= bnd), (rel, bnd)
+
+
+@pytest.mark.parametrize("case,B", [("ring42", 8), ("h60", 8)])
+@pytest.mark.parametrize("prec", ["bf16x6", "fp16x3"])
+def test_mixed_precision_large_configs(cuda, case, B, prec):
+    """The mixed paths of the 42-electron ring (Psiformer) and H60 (FermiNet,
+    60x60 FP64 determinants) against the FP64 oracle, with the kappa-scaled
+    per-walker bounds of the FP32 path (entry error of the mode)."""
+    import torch
+
+    sysm, spec, p, x, ref = _setup(case, B)
+    dm = _model(sysm, spec, p, prec=prec)
+    xe = torch.from_numpy(x).to(cuda)
+    lp = dm.logpsi(xe)[0]
+    out = dm.local_energy(xe, with_grad=True)
+    torch.cuda.synchronize()
+    d_lp = np.abs(lp.cpu().numpy() - ref["logpsi"])
+    assert np.all(d_lp <= logpsi_bound(ref["logpsi"], ref["kappa"], prec)), (d_lp, ref["kappa"])
+    rel, bnd = el_report(f"{case} B={B}", out["e_loc"].cpu().numpy(), ref["e_loc"], ref["kappa"], prec)
+    assert np.median(rel) <= 2e-5
+    assert np.all(rel <= bnd), (rel, bnd)
 
 
 def test_empty_batch_is_a_no_op(cuda):
@@ -322,5 +379,5 @@ def test_psiformer_chunking_and_bf16x6(cuda):
     oa, ob, oc = a.local_energy(xe), b.local_energy(xe), c.local_energy(xe)
     torch.cuda.synchronize()
     assert torch.equal(oa["e_loc"], ob["e_loc"])
-    rel = np.abs(oc["e_loc"].cpu().numpy() - ref["e_loc"]) / np.maximum(1.0, np.abs(ref["e_loc"]))
-    assert np.median(rel) < 2e-5 and rel.max() < 2e-3
+    rel, bnd = el_report("psi_small B=8", oc["e_loc"].cpu().numpy(), ref["e_loc"], ref["kappa"], "bf16x6")
+    assert np.median(rel) < 2e-5 and np.all(rel <= bnd)

@@ -87,3 +87,63 @@ def test_device_model_fails_loudly_without_gpu():
     sysm, spec, *_ = S.workload("li")
     with pytest.raises(_lib.JqError):
         DeviceModel(spec, sysm.atoms, sysm.charges, P.init_params(spec))
+
+
+def _desc(spec, atoms, charges, precision="fp32", ln="pre"):
+    d = _lib.JqModelDesc()
+    d.kind = _lib.KIND[spec.kind]
+    d.n_up, d.n_dn = spec.nspins
+    d.n_atoms = atoms.shape[0]
+    d.ndets = spec.ndets
+    d.n_layers = len(spec.hidden_single) if spec.kind == "ferminet" else spec.num_layers
+    for i, (a, b) in enumerate(zip(spec.hidden_single, spec.hidden_double)):
+        d.hidden_single[i], d.hidden_double[i] = a, b
+    d.num_heads, d.heads_dim, d.n_mlp = spec.num_heads, spec.heads_dim, len(spec.mlp_hidden)
+    d.envelope = _lib.ENVELOPE[spec.envelope]
+    d.precision = _lib.PRECISION[precision]
+    d.layer_norm_mode = _lib.LAYER_NORM[ln]
+    d.options = spec.options()
+    d.atoms = atoms.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    d.charges = charges.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    return d
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(use_last_layer=True), b"use_last_layer"),
+    (dict(envelope="diagonal"), b"diagonal"),
+    (dict(bias_orbitals=True), b"Psiformer only"),
+])
+def test_unsupported_options_rejected_before_allocation(change, msg):
+    """jq_model_create returns JQ_EINVAL for reference options the device path
+    does not implement, before touching the device (runs without a GPU)."""
+    lib = _lib.load()
+    sysm, spec, *_ = S.workload("li")
+    for k, v in change.items():
+        setattr(spec, k, v)
+    if "bias_orbitals" in change:  # a FermiNet with an orbital bias: the reference never builds one
+        spec.kind = "ferminet"
+    atoms = np.ascontiguousarray(sysm.atoms, np.float32)
+    charges = np.ascontiguousarray(sysm.charges, np.float32)
+    d = _desc(spec, atoms, charges)
+    if "bias_orbitals" in change:
+        d.options = _lib.OPT_ORBITAL_BIAS
+    flat = np.zeros(16, np.float32)
+    h = ctypes.c_void_p()
+    rc = lib.jq_model_create(ctypes.byref(d), flat.ctypes.data_as(ctypes.c_void_p), flat.size, ctypes.byref(h))
+    assert rc == -1
+    assert msg in lib.jq_last_error()
+    assert not h.value
+
+
+def test_bench_gemm_broadcast_rows_stay_in_bounds():
+    """jq_bench_gemm sizes its broadcast term for units // 7 + 2 walkers; the
+    unit -> walker map of its 7 + 7 electron layout (csrc/common.h
+    walker_of_unit) never reaches past that for any unit count."""
+    def walker_of_unit(u, Bc, n_up, n_dn):
+        uu = Bc * n_up
+        return u // n_up if u < uu else (u - uu) // n_dn
+
+    for units in range(1, 600):
+        Bc = units // 14 if units // 14 > 0 else 1
+        top = max(walker_of_unit(u, Bc, 7, 7) for u in range(units))
+        assert top < units // 7 + 2

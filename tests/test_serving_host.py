@@ -52,7 +52,8 @@ def test_admission_respects_memory_budget_and_fifo():
     clock = FakeClock()
     FakeRunner.log = []
     fp = lambda job: job.walkers  # bytes == walkers for the test
-    srv = serving.Server(lambda j: FakeRunner(j, clock), mem_budget=5000, max_active=8, footprint=fp, clock=clock)
+    srv = serving.Server(lambda j: FakeRunner(j, clock), mem_budget=5000, max_active=8, footprint=fp, clock=clock,
+                         async_build=False)
     for i, (w, s) in enumerate([(4096, 3), (1024, 1), (256, 1), (4096, 2)]):
         srv.submit(_job(i, w, s))
     ticks = 0
@@ -88,7 +89,8 @@ def test_timed_arrivals_are_admitted_when_they_arrive():
             return self.t
 
     tc = TickClock()
-    srv = serving.Server(lambda j: FakeRunner(j, tc), mem_budget=10 ** 9, footprint=lambda j: 1, clock=tc)
+    srv = serving.Server(lambda j: FakeRunner(j, tc), mem_budget=10 ** 9, footprint=lambda j: 1, clock=tc,
+                         async_build=False)
     jobs = [_job(i, 256, 2, arrival=float(i)) for i in range(4)]
     res = srv.run(jobs, idle_sleep=0)
     assert set(res) == {0, 1, 2, 3}
@@ -113,3 +115,21 @@ def test_footprint_grows_with_walkers_and_electrons():
     s2 = ModelSpec("ferminet", (21, 21), 8, ndets=16)
     assert serving.footprint_bytes(s1, 2, 1024) < serving.footprint_bytes(s1, 2, 4096)
     assert serving.footprint_bytes(s1, 2, 1024) < serving.footprint_bytes(s2, 8, 1024)
+
+
+def test_builder_thread_gives_the_same_results():
+    """Runners built by the builder threads join the batch when ready; the
+    results are those of synchronous construction."""
+    out = {}
+    for async_build in (False, True):
+        clock = FakeClock()
+        srv = serving.Server(lambda j: FakeRunner(j, clock), mem_budget=10 ** 9, max_active=3,
+                             footprint=lambda j: 1, clock=clock, async_build=async_build)
+        for i in range(7):
+            srv.submit(_job(i, 64, 1 + i % 3))
+        while srv.tick():
+            clock.t += 1.0
+        srv.close()
+        out[async_build] = {k: (r.energy, r.count) for k, r in srv.results.items()}
+        assert srv.peak_active <= 3
+    assert out[False] == out[True] and len(out[True]) == 7
