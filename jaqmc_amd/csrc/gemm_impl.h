@@ -595,14 +595,15 @@ bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
 // Default (UPT, NBW) per unit height and precision, from tile sweeps on
 // MI355X (K = 256-320, 256 output columns).  BF16-split modes at 48-row units
 // take four units per tile (the weight fragments of a chunk are reused over
-// 12 row blocks), FP32 two (three waves per SIMD).
+// 12 row blocks), FP32 and FP16X3 two (four units leave FP16X3 at two waves
+// per SIMD).
 void default_cfg(int R, int prec, bool wide, int* upt, int* nbw) {
   (void)wide;
   *nbw = 2;
   switch (R) {
     case 16: *upt = 4; break;
     case 32: *upt = 2; break;
-    case 48: *upt = prec == PREC_F32 ? 2 : 4; break;
+    case 48: *upt = (prec == PREC_F32 || prec == PREC_FP16X3) ? 2 : 4; break;
     case 64: *upt = 2; break;
     default: *upt = 1; break;
   }

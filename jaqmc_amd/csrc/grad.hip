@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) void det_vjp_kernel(FermiDims d, const float* 
   __shared__ double M[GJ_NP][2 * GJ_NP + 1];
   __shared__ double red_v[256];
   __shared__ int red_i[256];
+  __shared__ double mult[GJ_NP];  // column p of the rows, read before the update
   __shared__ double wk_s;
   const int w = blockIdx.x / d.K, k = blockIdx.x % d.K;
   const int N = d.N, KN = d.K * d.N;
@@ -102,10 +103,12 @@ __global__ __launch_bounds__(256) void det_vjp_kernel(FermiDims d, const float* 
     const double inv = 1.0 / M[p][p];
     __syncthreads();
     for (int j = tid; j < 2 * N; j += 256) M[p][j] *= inv;
+    for (int r = tid; r < N; r += 256)
+      if (r != p) mult[r] = M[r][p];
     __syncthreads();
     for (int e = tid; e < N * 2 * N; e += 256) {
       const int r = e / (2 * N), j = e % (2 * N);
-      if (r != p) M[r][j] -= M[r][p] * M[p][j];
+      if (r != p) M[r][j] -= mult[r] * M[p][j];
     }
     __syncthreads();
   }

@@ -79,8 +79,18 @@ class JobResult:
         return self.finished - self.arrival
 
 
+JOB_WORKSPACE = 4 << 30  # bytes of walker-chunk workspace per job
+
+
+def job_chunk(spec: ModelSpec, n_atoms: int, walkers: int) -> int:
+    """Walkers per workspace chunk of a job: the whole job when it fits
+    JOB_WORKSPACE, else as many as fit."""
+    per = _per_walker_bytes(spec, n_atoms)
+    return int(max(1, min(walkers, JOB_WORKSPACE // per)))
+
+
 def footprint_bytes(spec: ModelSpec, n_atoms: int, walkers: int, precision: str = "fp32") -> int:
-    """Device bytes of a job: weights + a one-chunk workspace + walker state.
+    """Device bytes of a job: weights + its chunk workspace + walker state.
 
     Mirrors the library's FermiNet workspace sizing (jq_model::init_ferminet):
     two tangent buffers of N*Rp rows, the spin-mean and broadcast terms, the
@@ -88,12 +98,21 @@ def footprint_bytes(spec: ModelSpec, n_atoms: int, walkers: int, precision: str 
     """
     from .params import init_params, pack
 
+    N = spec.n_elec
+    T = 3 * N
+    K = spec.ndets
+    weights = pack(spec, init_params(spec, seed=0)).size * 4 * (4 if precision != "fp32" else 1)
+    state = walkers * (T * 4 * 2 + 4 * 2 + 4 + 8 * 4)
+    per = _per_walker_bytes(spec, n_atoms)
+    return int(weights + per * job_chunk(spec, n_atoms, walkers) + state + (32 << 20))
+
+
+def _per_walker_bytes(spec: ModelSpec, n_atoms: int) -> int:
     r32 = lambda v: (v + 31) // 32 * 32
     N = spec.n_elec
     T = 3 * N
     Rp = next(o for o in (16, 32, 48, 64, 96, 128, 192) if T + 2 <= o)
     K = spec.ndets
-    weights = pack(spec, init_params(spec, seed=0)).size * 4 * (4 if precision != "fp32" else 1)
     ldphi = (K * N + 3) // 4 * 4
     if spec.kind == "ferminet":
         nch = spec.n_channels
@@ -111,8 +130,7 @@ def footprint_bytes(spec: ModelSpec, n_atoms: int, walkers: int, precision: str 
         H = spec.num_heads
         per = 4 * (7 * u * D + 2 * H * u * ((N + 3) // 4 * 4) + H * u * r32(N) + H * N * r32(u) + u * ldphi
                    + N * K * N * 5) + 8 * K * (3 + T) + 64
-    state = walkers * (T * 4 * 2 + 4 * 2 + 4 + 8 * 4)
-    return int(weights + per * walkers + state + (32 << 20))
+    return int(per)
 
 
 class Server:
@@ -237,7 +255,7 @@ class GpuRunner:
         self.stream = torch.cuda.Stream()
         with torch.cuda.stream(self.stream):
             self.model = DeviceModel(job.spec, job.atoms, job.charges, job.params, precision=precision,
-                                     chunk_walkers=job.walkers)
+                                     chunk_walkers=job_chunk(job.spec, job.atoms.shape[0], job.walkers))
             self.x = torch.from_numpy(np.ascontiguousarray(job.x0, np.float32)).to("cuda", non_blocking=False)
             self.logp = 2.0 * self.model.logpsi(self.x)[0]
             self.acc = torch.zeros(job.walkers, dtype=torch.int32, device="cuda")
@@ -302,3 +320,47 @@ def latency_summary(results: Dict[int, JobResult]):
     t1 = max(r.finished for r in results.values())
     return {"jobs": len(lat), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "walker_steps_per_s": ws / (t1 - t0), "makespan_s": t1 - t0}
+
+
+def sweep(rates, n_jobs, steps=100, walkers=(256, 1024, 4096), precision="fp16x3", seed=0, max_active=8,
+          mem_budget=200 << 30, log=None):
+    """Serve a seeded Poisson stream of random-molecule jobs at each arrival
+    rate and summarise latency and throughput (the serving workload of the
+    benchmark: 2-42 electrons, FermiNet 4x256/32, 16 determinants).  With
+    fewer than 100 jobs per rate the p99 is reported together with the
+    maximum it approaches."""
+    out = []
+    for r in rates:
+        jobs = make_job_stream(n_jobs, rate=r, seed=seed, steps=steps, walkers=walkers)
+        t0 = time.perf_counter()
+        for j in jobs:
+            j.arrival += t0
+        srv = Server(lambda j: GpuRunner(j, precision=precision), mem_budget=mem_budget, max_active=max_active)
+        res = srv.run(jobs)
+        srv.close()
+        s = latency_summary(res)
+        lat = np.array([x.latency for x in res.values()])
+        s.update({"rate_jobs_per_s": r, "p90_s": float(np.percentile(lat, 90)), "max_s": float(lat.max()),
+                  "p99_is_near_max": n_jobs < 100, "peak_active": srv.peak_active,
+                  "walker_steps": int(sum(x.walkers * x.steps for x in res.values()))})
+        out.append(s)
+        if log:
+            log(s)
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description="serving sweep over Poisson arrival rates")
+    ap.add_argument("--rates", default="1,5,50")
+    ap.add_argument("--jobs", type=int, default=24)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--precision", default="fp16x3")
+    ap.add_argument("--max-active", type=int, default=8)
+    a = ap.parse_args()
+    res = sweep([float(r) for r in a.rates.split(",")], a.jobs, steps=a.steps, precision=a.precision,
+                max_active=a.max_active, log=lambda s: print(json.dumps(s), file=sys.stderr, flush=True))
+    print(json.dumps(res))

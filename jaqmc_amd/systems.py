@@ -3,10 +3,14 @@
 Walker initialisation restates utils/atomic/initialization.py:124-163: each
 electron starts on an assigned nucleus plus ``width * N(0, 1)`` noise, with the
 electron order all spin-up electrons (atom 0, atom 1, ...) then all spin-down
-electrons (atom 0, atom 1, ...).  The per-atom spin assignment of
-``distribute_spins`` (initialization.py:39-122) is replaced by a deterministic
-alternation for odd-electron atoms (documented; it only affects where walkers
-start, not the wavefunction).
+electrons (atom 0, atom 1, ...).  The per-atom spins follow ``distribute_spins``
+(initialization.py:39-122) without per-atom hints: every atom starts from its
+chemical default (n_alpha, n_beta) = ((Z + u) / 2, (Z - u) / 2) with u the
+ground-state unpaired-electron count of the element (utils/atomic/atom.py:57-79),
+then single electrons move between the channels of randomly chosen atoms until
+the target (n_up, n_dn) is met.  The random choice uses a seeded numpy
+generator (JAX's threefry stream is not reproduced); it only affects where
+walkers start.
 """
 
 from __future__ import annotations
@@ -28,35 +32,32 @@ class System:
     spins_per_atom: List[Tuple[int, int]]
 
 
-def _spins_per_atom(charges, nspins):
-    """Neutral atoms; even Z split evenly, odd Z alternate the extra electron
-    between spin channels until the target (n_up, n_dn) is met."""
+# Ground-state unpaired electrons of the neutral atoms H .. Ne (Hund's rule)
+UNPAIRED = {1: 1, 2: 0, 3: 1, 4: 0, 5: 1, 6: 2, 7: 3, 8: 2, 9: 1, 10: 0}
+
+
+def _spins_per_atom(charges, nspins, seed=0):
+    """distribute_spins (initialization.py:39-122) for neutral atoms without
+    hints: chemical defaults, then random single-electron moves between the
+    channels of candidate atoms until the target (n_up, n_dn) is met."""
     out = []
-    extra_up = True
     for z in charges:
         z = int(round(float(z)))
-        up, dn = z // 2, z // 2
-        if z % 2:
-            if extra_up:
-                up += 1
-            else:
-                dn += 1
-            extra_up = not extra_up
-        out.append((up, dn))
-    # fix totals to nspins (move electrons between channels on the same atom)
+        u = UNPAIRED[z]
+        out.append(((z + u) // 2, (z - u) // 2))
+    if sum(a + b for a, b in out) != sum(nspins):
+        raise ValueError(f"{sum(a + b for a, b in out)} electrons on the atoms, nspins {nspins}")
+    rng = np.random.default_rng(seed)
     cu = sum(a for a, _ in out)
-    i = 0
-    while cu != nspins[0] and i < 4 * len(out):
-        j = i % len(out)
-        a, b = out[j]
-        if cu > nspins[0] and a > 0:
-            out[j] = (a - 1, b + 1)
-            cu -= 1
-        elif cu < nspins[0] and b > 0:
-            out[j] = (a + 1, b - 1)
-            cu += 1
-        i += 1
-    assert cu == nspins[0], (out, nspins)
+    while cu != nspins[0]:
+        up = nspins[0] > cu  # move an electron from beta to alpha
+        cand = [i for i, (a, b) in enumerate(out) if (b if up else a) > 0]
+        if not cand:
+            raise ValueError(f"cannot reach spins {nspins} from {out}")
+        i = cand[int(rng.integers(0, len(cand)))]
+        a, b = out[i]
+        out[i] = (a + 1, b - 1) if up else (a - 1, b + 1)
+        cu += 1 if up else -1
     return out
 
 

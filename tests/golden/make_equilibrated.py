@@ -30,7 +30,7 @@ from jaqmc_amd import params as P  # noqa: E402
 from jaqmc_amd import systems as S  # noqa: E402
 
 # workload: (walkers, equilibration steps, starting width)
-PLAN = {"li": (256, 300, 0.3), "n2": (256, 300, 0.1), "ring42": (96, 200, 0.08), "h60": (64, 200, 0.15)}
+PLAN = {"li": (256, 300, 0.3), "n2": (256, 300, 0.1), "ring42": (256, 200, 0.08), "h60": (128, 200, 0.15)}
 
 
 def _setup(name):

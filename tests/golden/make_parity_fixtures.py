@@ -5,7 +5,8 @@ Each case is a workload (jaqmc_amd.systems.workload) with parameters from
 params.init_params(seed=11, bias_scale=0.1, env_jitter=0.2) and walkers from
 systems.init_walkers(seed=12); the file holds the walkers and the float64
 oracle's log|psi|, sign, local energy (kinetic, potential, total), gradient
-and each walker's largest orbital-matrix 2-norm condition number.  The GPU
+each walker's largest orbital-matrix 2-norm condition number and the
+cancellation factor of its determinant sum (sum_k |det| / |sum_k det|).  The GPU
 tests regenerate parameters and walkers from the same seeds and check them
 against the stored copy before comparing.
 """
@@ -42,7 +43,10 @@ def _one(args):
     ref = ham.local_energy(spec.as_oracle(), p64, x.astype(np.float64), at, ch)
     A = np.asarray(owf.orbital_matrices(spec.as_oracle(), p64, x.astype(np.float64), at))
     ref["kappa"] = np.linalg.cond(A).reshape(x.shape[0], -1).max(axis=1)
-    return {k: np.asarray(ref[k]) for k in ("logpsi", "sign", "e_loc", "e_kin", "e_pot", "grad", "kappa")}
+    s, l = np.linalg.slogdet(A)  # cancellation factor of the determinant sum
+    m = l.max(axis=1, keepdims=True)
+    ref["cancel"] = np.exp(l - m).sum(1) / np.abs((s * np.exp(l - m)).sum(1))
+    return {k: np.asarray(ref[k]) for k in ("logpsi", "sign", "e_loc", "e_kin", "e_pot", "grad", "kappa", "cancel")}
 
 
 def main():

@@ -53,7 +53,9 @@ def test_loss_clip_matches_numpy(cuda, case, method):
     lo, hi = st[2], st[3]
     np.testing.assert_array_equal(c.cpu().numpy(), np.where(x < lo, lo, np.where(x > hi, hi, x)))
     assert st[4] == rst[4]
-    np.testing.assert_allclose(st[:2], rst[:2], rtol=1e-12)
+    # the means: FP64 sums in a different order (Cauchy samples cancel strongly)
+    scale = np.nanmean(np.abs(np.where(np.isfinite(x), x, 0.0))) + 1.0
+    np.testing.assert_allclose(st[:2], rst[:2], rtol=1e-12, atol=1e-12 * scale)
     np.testing.assert_allclose(w.cpu().numpy(), rw, rtol=1e-9, atol=1e-15)
 
 

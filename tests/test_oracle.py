@@ -72,7 +72,7 @@ def test_forward_laplacian_matches_finite_differences(name, kw):
     m = spec.as_oracle()
     _, lp = wf.logpsi(m, p, x, atoms, with_laplacian=True)
     f = lambda y: wf.logpsi(m, p, y, atoms, with_laplacian=False)[1].v
-    g, L = _fd_grad_lap(f, x, 2e-3)
+    g, L = _fd_grad_lap(f, x, 1e-3)
     np.testing.assert_allclose(lp.v, f(x), rtol=1e-12)
     np.testing.assert_allclose(lp.J, g, rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(lp.lap, L, rtol=2e-5, atol=1e-4)
