@@ -143,8 +143,9 @@ class Server:
 
     def __init__(self, runner_factory: Callable, mem_budget: int, max_active: int = 8,
                  footprint: Callable = None, clock: Callable = time.perf_counter, async_build: bool = True,
-                 builders: int = 2):
+                 builders: int = 2, on_done: Optional[Callable] = None):
         self.factory = runner_factory
+        self.on_done = on_done
         self.async_build = async_build
         self._pool = cf.ThreadPoolExecutor(builders) if async_build else None
         self.starting: List[dict] = []
@@ -212,6 +213,8 @@ class Server:
                                                      self.clock(), job.spec.n_elec, job.walkers, job.steps)
                 a["runner"].close()
                 self.mem_used -= a["need"]
+                if self.on_done:
+                    self.on_done(self.results[job.job_id])
             else:
                 still.append(a)
         self.active = still
@@ -323,7 +326,7 @@ def latency_summary(results: Dict[int, JobResult]):
 
 
 def sweep(rates, n_jobs, steps=100, walkers=(256, 1024, 4096), precision="fp16x3", seed=0, max_active=8,
-          mem_budget=200 << 30, log=None):
+          mem_budget=200 << 30, log=None, progress=None):
     """Serve a seeded Poisson stream of random-molecule jobs at each arrival
     rate and summarise latency and throughput (the serving workload of the
     benchmark: 2-42 electrons, FermiNet 4x256/32, 16 determinants).  With
@@ -335,7 +338,10 @@ def sweep(rates, n_jobs, steps=100, walkers=(256, 1024, 4096), precision="fp16x3
         t0 = time.perf_counter()
         for j in jobs:
             j.arrival += t0
-        srv = Server(lambda j: GpuRunner(j, precision=precision), mem_budget=mem_budget, max_active=max_active)
+        done = (lambda x: progress(f"rate {r}: job {x.job_id} ({x.n_elec} e, {x.walkers} walkers) "
+                                   f"latency {x.latency:.2f} s")) if progress else None
+        srv = Server(lambda j: GpuRunner(j, precision=precision), mem_budget=mem_budget, max_active=max_active,
+                     on_done=done)
         res = srv.run(jobs)
         srv.close()
         s = latency_summary(res)
@@ -362,5 +368,6 @@ if __name__ == "__main__":
     ap.add_argument("--max-active", type=int, default=8)
     a = ap.parse_args()
     res = sweep([float(r) for r in a.rates.split(",")], a.jobs, steps=a.steps, precision=a.precision,
-                max_active=a.max_active, log=lambda s: print(json.dumps(s), file=sys.stderr, flush=True))
+                max_active=a.max_active, log=lambda s: print(json.dumps(s), file=sys.stderr, flush=True),
+                progress=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(res))
