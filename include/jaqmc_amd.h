@@ -209,6 +209,20 @@ size_t jq_loss_clip_workspace_bytes(int B);
 int jq_loss_clip(const double* x, int B, int method, double scale, double* clipped, double* weights,
                  double* stats, void* workspace, size_t ws_bytes, void* stream);
 
+/* jq_loss_clip without a workspace argument.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops or harnesses that forbid hipMalloc).  Same kernels and
+ * results as jq_loss_clip; the scratch is a static device array of
+ * JQ_LOSS_CLIP_SCRATCH_BYTES per GPU (one copy per device, held until the
+ * process ends), so the call allocates nothing.  B above
+ * JQ_LOSS_CLIP_NOALLOC_MAX_B is rejected with JQ_EINVAL.  All calls on one GPU
+ * share that array and every call rewrites it before reading it: calls that
+ * overlap on one GPU (two streams or two host threads at once) race. */
+#define JQ_LOSS_CLIP_NOALLOC_MAX_B (1 << 28)
+#define JQ_LOSS_CLIP_SCRATCH_BYTES 262464
+int jq_loss_clip_noalloc(const double* x, int B, int method, double scale, double* clipped, double* weights,
+                         double* stats, void* stream);
+
 const char* jq_last_error(void);
 
 #ifdef __cplusplus

@@ -30,7 +30,7 @@ EXPORTS = (
     "jq_model_create", "jq_model_destroy", "jq_model_workspace_bytes", "jq_logpsi", "jq_mh_steps",
     "jq_local_energy", "jq_reduce_mean_var", "jq_reduce_accepts", "jq_last_error",
     "jq_profile_enable", "jq_profile_read", "jq_orbitals", "jq_sample", "jq_loss_clip",
-    "jq_loss_clip_workspace_bytes", "jq_logpsi_vjp", "jq_model_param_count",
+    "jq_loss_clip_workspace_bytes", "jq_logpsi_vjp", "jq_model_param_count", "jq_loss_clip_noalloc",
 )
 
 # launch classes of include/jaqmc_amd_prof.h
@@ -94,6 +94,7 @@ def load():
     lib.jq_loss_clip_workspace_bytes.argtypes = [i32]
     lib.jq_loss_clip_workspace_bytes.restype = sz
     lib.jq_loss_clip.argtypes = [vp, i32, i32, f64, vp, vp, vp, vp, sz, vp]
+    lib.jq_loss_clip_noalloc.argtypes = [vp, i32, i32, f64, vp, vp, vp, vp]
     lib.jq_logpsi_vjp.argtypes = [vp, vp, i32, vp, vp, vp]
     lib.jq_model_param_count.argtypes = [vp]
     lib.jq_model_param_count.restype = sz

@@ -18,9 +18,11 @@
 // depends on the workgroup count or order, so results are deterministic.
 //
 // The scratch buffer (partial histograms + selection state) is a caller
-// workspace of jq_loss_clip_workspace_bytes() bytes.
+// workspace of jq_loss_clip_workspace_bytes() bytes, or, for
+// jq_loss_clip_noalloc, a static device array of the same size.
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../include/jaqmc_amd.h"
@@ -52,6 +54,7 @@ struct ClipState {
 };
 // Scratch layout: [kSel Sel][ClipState][kHistBlocks x 256 uint32 partials]
 constexpr size_t kScratchBytes = kSel * sizeof(Sel) + sizeof(ClipState) + (size_t)kHistBlocks * 256 * 4;
+static_assert(kScratchBytes == JQ_LOSS_CLIP_SCRATCH_BYTES, "scratch size of the header");
 
 __device__ inline unsigned long long key_of(double v) {
   const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
@@ -244,6 +247,25 @@ int loss_clip(const double* x, int B, int method, double scale, double* clipped,
   return 0;
 }
 
+// The static scratch of jq_loss_clip_noalloc: one copy per device, held until
+// the process ends.  Host code never uses the array's name as an address; it
+// uses the device address from hipGetSymbolAddress, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static void* scratch_address() {
+  static std::mutex mu;
+  static void* ptr[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+
 int check_args(const double* x, int B, int method, double* weights) {
   if (B < 1 || B > JQ_LOSS_CLIP_MAX_B || !x || !weights || method < 0 || method > 2)
     return jq_fail(JQ_EINVAL, "jq_loss_clip: bad argument (1 <= B <= JQ_LOSS_CLIP_MAX_B, method 0 mad / 1 iqr / 2 none)");
@@ -267,6 +289,19 @@ int jq_loss_clip(const double* x, int B, int method, double scale, double* clipp
     return jq_fail(JQ_EINVAL, "jq_loss_clip: workspace smaller than jq_loss_clip_workspace_bytes()");
   loss_clip(x, B, method, scale, clipped, weights, stats, workspace, (hipStream_t)stream);
   if (hipGetLastError() != hipSuccess) return jq_fail(JQ_EHIP, "kernel launch failed in jq_loss_clip");
+  return JQ_OK;
+}
+
+int jq_loss_clip_noalloc(const double* x, int B, int method, double scale, double* clipped, double* weights,
+                         double* stats, void* stream) {
+  int rc = check_args(x, B, method, weights);
+  if (rc) return rc;
+  if (B > JQ_LOSS_CLIP_NOALLOC_MAX_B)
+    return jq_fail(JQ_EINVAL, "jq_loss_clip_noalloc: B exceeds JQ_LOSS_CLIP_NOALLOC_MAX_B");
+  void* p = scratch_address();
+  if (!p) return jq_fail(JQ_EHIP, "jq_loss_clip_noalloc: device address of the static scratch not found");
+  loss_clip(x, B, method, scale, clipped, weights, stats, p, (hipStream_t)stream);
+  if (hipGetLastError() != hipSuccess) return jq_fail(JQ_EHIP, "kernel launch failed in jq_loss_clip_noalloc");
   return JQ_OK;
 }
 

@@ -212,6 +212,22 @@ def loss_clip(x, method="mad", scale=5.0, workspace=None):
     return clipped, w, stats
 
 
+def loss_clip_noalloc(x, method="mad", scale=5.0, out=None):
+    """loss_clip through jq_loss_clip_noalloc: the library allocates nothing
+    (its scratch is a static device array per GPU); out = (clipped, weights,
+    stats) preallocated tensors, or None to allocate them here."""
+    lib = _lib.load()
+    B = x.numel()
+    if out is None:
+        out = (torch.empty(B, dtype=torch.float64, device=x.device),
+               torch.empty(B, dtype=torch.float64, device=x.device),
+               torch.empty(5, dtype=torch.float64, device=x.device))
+    clipped, w, stats = out
+    _lib.check(lib.jq_loss_clip_noalloc(_ptr(x), B, CLIP_METHODS[method], float(scale), _ptr(clipped), _ptr(w),
+                                        _ptr(stats), _stream()), "jq_loss_clip_noalloc")
+    return clipped, w, stats
+
+
 def reduce_accepts(n_accept):
     lib = _lib.load()
     out = torch.empty(1, dtype=torch.float64, device=n_accept.device)
