@@ -94,18 +94,27 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // (8 consecutive k of one column) straight from global memory, where the
   // weights stay L2-resident.
   constexpr int A_BYTES = NSL * BM * LS * (int)sizeof(TL);
+  // Two operand tiles when they fit 48 KB (three workgroups per CU): chunk
+  // it+1 is staged into one while chunk it is multiplied from the other, so
+  // the K loop needs one barrier per chunk.  Larger tiles use one buffer and
+  // two barriers.
+  constexpr bool DB = 2 * A_BYTES <= 48 * 1024;
+  constexpr int NBUF = DB ? 2 : 1;
   constexpr int STG_BYTES = 4 * 16 * (NBW * 16 + 4) * 4;
-  constexpr int ARENA = A_BYTES > STG_BYTES ? A_BYTES : STG_BYTES;
+  constexpr int ARENA = NBUF * A_BYTES > STG_BYTES ? NBUF * A_BYTES : STG_BYTES;
   __shared__ __attribute__((aligned(16))) char arena[ARENA];
-  TL (*As)[BM * LS] = (TL (*)[BM * LS])arena;
+  constexpr int PLANE = BM * LS;  // elements of one operand plane
+  TL* const Abase = (TL*)arena;
   // PREC_FP16X3: power-of-two scale of each tile row (1 in the other modes)
   __shared__ float rsc[F16 ? BM : 1], rsi[F16 ? BM : 1];
 
   // register prefetch: A chunk as float4 (8 per 32-wide row); B fragments
   constexpr int AV = (BM * 8 + 255) / 256;
-  float4 pa[AV];
-  float4 bn32[NBW][2];  // FP32: k = 8g .. 8g+7 of column nb
-  uint4 bnb[NS][NBW];   // BF16 planes: the same 8 k as bf16x8
+  f32x4 pa[AV];
+  // B fragments, in two sets used alternately by consecutive chunks (see the
+  // K loop); set 1 is used only where EARLY_B holds.
+  f32x4 bn32[2][NBW][2];  // FP32: k = 8g .. 8g+7 of column nb
+  uint4 bnb[2][NS][NBW];   // 16-bit planes: the same 8 k as 8 halves
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -129,42 +138,48 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   const TL* Wb = F16 ? (const TL*)((const bf16*)a.Wt_lo + 3 * a.wplane) : (const TL*)a.Wt_lo;
   const float* csinv = F16 ? (const float*)(Wb + 2 * a.wplane) : nullptr;
 
+  // Operand loads are unconditional: rows past the end read the last row and
+  // columns past the end read column 0, so no load sits behind a branch (the
+  // compiler then counts outstanding loads exactly).  Such rows and columns
+  // only reach accumulators that are never stored: rows are independent,
+  // Laplacian-mode units are whole, and output columns >= ncols are skipped.
   auto gload = [&](int k0) __attribute__((always_inline)) {
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int v = 0; v < AV; ++v) {
-      const int idx = tid + v * 256;
+      int idx = tid + v * 256;
+      if (idx >= BM * 8) idx -= BM * 8;  // duplicate of an in-tile load (not staged)
       const int r = idx >> 3, q = idx & 7;
-      const long gr = row0 + r;
-      pa[v] = (idx < BM * 8 && gr < a.rows) ? *(const float4*)(X + gr * a.ldx + k0 + q * 4)
-                                             : float4{0.f, 0.f, 0.f, 0.f};
+      const long gr = row0 + r < a.rows ? row0 + r : (long)a.rows - 1;
+      pa[v] = *(const f32x4*)(X + gr * a.ldx + k0 + q * 4);
     }
   };
-  auto bload = [&](int k0) __attribute__((always_inline)) {
+  auto bload = [&](auto set, int k0) __attribute__((always_inline)) {
+    constexpr int S = decltype(set)::value;
 #pragma unroll
     for (int nb = 0; nb < NBW; ++nb) {
       const int gc = col0 + wcol + nb * 16 + lr;
-      const bool ok = gc < a.ncols;
+      const long gcl = gc < a.ncols ? gc : 0;
       if constexpr (F32) {
-        const float* p = W + (long)(ok ? gc : 0) * a.ldw + k0 + 8 * lg;
-        bn32[nb][0] = ok ? *(const float4*)p : float4{0.f, 0.f, 0.f, 0.f};
-        bn32[nb][1] = ok ? *(const float4*)(p + 4) : float4{0.f, 0.f, 0.f, 0.f};
+        const float* p = W + gcl * a.ldw + k0 + 8 * lg;
+        bn32[S][nb][0] = *(const f32x4*)p;
+        bn32[S][nb][1] = *(const f32x4*)(p + 4);
       } else {
 #pragma unroll
-        for (int s = 0; s < NS; ++s)
-          bnb[s][nb] = ok ? *(const uint4*)(Wb + s * a.wplane + (long)gc * a.ldw + k0 + 8 * lg) : uint4{0u, 0u, 0u, 0u};
+        for (int s = 0; s < NS; ++s) bnb[S][s][nb] = *(const uint4*)(Wb + s * a.wplane + gcl * a.ldw + k0 + 8 * lg);
       }
     }
   };
-  auto lstore = [&]() __attribute__((always_inline)) {
-#pragma unroll
+  auto lstore = [&](int buf) __attribute__((always_inline)) {
+    TL* const Ab = Abase + buf * NSL * PLANE;
+#pragma clang loop unroll(full)
     for (int v = 0; v < AV; ++v) {
       const int idx = tid + v * 256;
-      if (idx >= BM * 8) break;
       const int r = idx >> 3, q = idx & 7;
+      if ((v + 1) * 256 > BM * 8 && idx >= BM * 8) continue;  // only a last, partial pass
       if constexpr (F32) {
-        *(float4*)(&As[0][r * LS + q * 4]) = pa[v];
+        *(f32x4*)(&Ab[r * LS + q * 4]) = pa[v];
       } else {
-        float4 x = pa[v];
+        f32x4 x = pa[v];
         if constexpr (F16) {
           const float sc = rsc[r];
           x.x *= sc;
@@ -180,40 +195,46 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           TL h[4] = {t[0][s], t[1][s], t[2][s], t[3][s]};
-          *(uint2*)(&As[s][r * LS + q * 4]) = *(uint2*)h;
+          *(uint2*)(&Ab[s * PLANE + r * LS + q * 4]) = *(uint2*)h;
         }
       }
     }
   };
 
-  // PREC_FP16X3 row scales: the value and Laplacian rows of every unit
-  // (row % period 0 or 1) are scanned over K (one wave per row); other rows
-  // take the fixed Jacobian scale.
+  // PREC_FP16X3 row scales: every row takes the fixed Jacobian scale, then
+  // the value and Laplacian rows of every unit (row % period 0 or 1) are
+  // scanned over K, one wave per row.  The candidate rows of the tile are
+  // enumerated from one modulo per workgroup.
   float csi[NBW], csf[NBW];  // inverse and plain column scales of the lane's columns (FP16X3)
   if constexpr (F16) {
     const int per = LAP ? 16 * MBU : (a.lrp > 0 ? a.lrp : 1);
-    for (int r = wave; r < BM; r += 4) {
+    for (int r = tid; r < BM; r += 256) {
+      const bool in = row0 + r < a.rows;
+      rsc[r] = in ? kF16JScale : 1.f;
+      rsi[r] = in ? 1.f / kF16JScale : 1.f;
+    }
+    __syncthreads();
+    const int off = (int)(row0 % per);
+    const int cpr = per > 1 ? 2 : 1;  // scanned rows per period
+    const int ncand = cpr * ((BM + off + per - 1) / per);
+    for (int q = wave; q < ncand; q += 4) {
+      const int r = (q / cpr) * per + (q % cpr) - off;
       const long gr = row0 + r;
-      const int cls = (int)(gr % per);
-      float sc = kF16JScale, si = 1.f / kF16JScale;
-      if (gr >= a.rows) {
-        sc = si = 1.f;
-      } else if (cls <= 1) {
-        float mx = 0.f;
-        for (int k = lane * 4; k < a.K; k += 256) {
-          const float4 v = *(const float4*)(X + gr * a.ldx + k);
-          mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
-        }
-        mx = wave_max(mx);
-        sc = si = 1.f;
-        if (mx > 0.f && mx <= 3.0e38f) {
-          int e;
-          (void)frexpf(mx, &e);
-          int sh = 14 - e;
-          sh = sh < -60 ? -60 : (sh > 60 ? 60 : sh);
-          sc = ldexpf(1.f, sh);
-          si = ldexpf(1.f, -sh);
-        }
+      if (r < 0 || r >= BM || gr >= a.rows) continue;
+      float mx = 0.f;
+      for (int k = lane * 4; k < a.K; k += 256) {
+        const float4 v = *(const float4*)(X + gr * a.ldx + k);
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+      }
+      mx = wave_max(mx);
+      float sc = 1.f, si = 1.f;
+      if (mx > 0.f && mx <= 3.0e38f) {
+        int e;
+        (void)frexpf(mx, &e);
+        int sh = 14 - e;
+        sh = sh < -60 ? -60 : (sh > 60 ? 60 : sh);
+        sc = ldexpf(1.f, sh);
+        si = ldexpf(1.f, -sh);
       }
       if (lane == 0) {
         rsc[r] = sc;
@@ -289,82 +310,110 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     const int c = it + kst;
     return (c >= nk ? c - nk : c) * KC;
   };
-  gload(kof(0));
-  bload(kof(0));
-  for (int it = 0; it < nk; ++it) {
-    lstore();
-    // this chunk's B fragments.  FP32: the next chunk's are loaded while this
-    // one is multiplied.  BF16 modes: the NS planes are used from the load
-    // registers and the next chunk's are requested after the last MFMA (they
-    // are L2 hits and arrive during the next staging step), which keeps the
-    // 128-row tiles at three waves per SIMD.
-    float bf[NBW][8];
-    bf16x8 bt[NS][NBW];
-    if constexpr (F32) {
-#pragma unroll
-      for (int nb = 0; nb < NBW; ++nb) {
-        *(float4*)&bf[nb][0] = bn32[nb][0];
-        *(float4*)&bf[nb][4] = bn32[nb][1];
-      }
-    }
-    __syncthreads();
-    if (it + 1 < nk) {
-      gload(kof(it + 1));
-      if constexpr (F32) bload(kof(it + 1));
-    }
-    if constexpr (!F32) {
-#pragma unroll
-      for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int nb = 0; nb < NBW; ++nb) bt[s][nb] = __builtin_bit_cast(bf16x8, bnb[s][nb]);
-    }
-
+  // products of one chunk from operand tile `buf` with B fragment set `set`
+  auto mfma_chunk = [&](auto set, int buf) __attribute__((always_inline)) {
+    constexpr int S = decltype(set)::value;
+    const TL* const Ab = Abase + buf * NSL * PLANE;
     if constexpr (F32) {
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
-        float af[8];
-        const float* p = &As[0][(mb * 16 + lr) * LS + lg * 8];
-        *(float4*)&af[0] = *(const float4*)p;
-        *(float4*)&af[4] = *(const float4*)(p + 4);
+        const f32x4 a0 = *(const f32x4*)&Ab[(mb * 16 + lr) * LS + lg * 8];
+        const f32x4 a1 = *(const f32x4*)&Ab[(mb * 16 + lr) * LS + lg * 8 + 4];
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk)
 #pragma unroll
           for (int nb = 0; nb < NBW; ++nb)
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[kk], bf[nb][kk], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(kk < 4 ? a0[kk & 3] : a1[kk & 3],
+                                                               bn32[S][nb][kk >> 2][kk & 3], acc[mb][nb], 0, 0, 0);
       }
     } else {
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
         bf16x8 at[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) at[s] = *(const bf16x8*)&As[s][(mb * 16 + lr) * LS + lg * 8];
+        for (int s = 0; s < NS; ++s) at[s] = *(const bf16x8*)&Ab[s * PLANE + (mb * 16 + lr) * LS + lg * 8];
 #pragma unroll
         for (int nb = 0; nb < NBW; ++nb) {
           if constexpr (F16) {
             // small terms first
             const f16x8 a0 = __builtin_bit_cast(f16x8, at[0]), a1 = __builtin_bit_cast(f16x8, at[1]);
-            const f16x8 b0 = __builtin_bit_cast(f16x8, bt[0][nb]), b1 = __builtin_bit_cast(f16x8, bt[1][nb]);
+            const f16x8 b0 = __builtin_bit_cast(f16x8, bnb[S][0][nb]), b1 = __builtin_bit_cast(f16x8, bnb[S][1][nb]);
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b0, acc[mb][nb], 0, 0, 0);
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b1, acc[mb][nb], 0, 0, 0);
             acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b0, acc[mb][nb], 0, 0, 0);
             continue;
           }
+          bf16x8 bt[NS];
+#pragma unroll
+          for (int s = 0; s < NS; ++s) bt[s] = __builtin_bit_cast(bf16x8, bnb[S][s][nb]);
           // small terms first
           if constexpr (NS == 3) {
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[1][nb], acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[2][nb], acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[2], bt[0][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[1], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[2], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[2], bt[0], acc[mb][nb], 0, 0, 0);
           }
           if constexpr (NS >= 2) {
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[1][nb], acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[0][nb], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[1], acc[mb][nb], 0, 0, 0);
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[0], acc[mb][nb], 0, 0, 0);
           }
-          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0][nb], acc[mb][nb], 0, 0, 0);
+          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0], acc[mb][nb], 0, 0, 0);
         }
       }
-      if (it + 1 < nk) bload(kof(it + 1));
     }
+  };
+  // EARLY_B: the next chunk's B fragments are requested into the other set
+  // before this chunk's products, so no fragment is ever copied and no wait
+  // covers loads issued after them.  Otherwise (three planes or wide tiles,
+  // where two sets do not fit the register budget) one set is used and the
+  // next chunk's fragments are requested after the last product (L2 hits that
+  // arrive during the staging step).
+  constexpr bool EARLY_B = F32 || NS * NBW <= 4;
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+
+  gload(kof(0));
+  bload(S0{}, kof(0));
+  if constexpr (DB) {
+    // chunk it is multiplied from tile it&1 while chunk it+1 is staged into
+    // the other tile (its loads were issued one chunk earlier); the barrier
+    // at the end of a step publishes tile (it+1)&1 and retires every read of
+    // tile it&1 before step it+1 overwrites it.
+    lstore(0);
+    gload(kof(nk > 1 ? 1 : 0));
     __syncthreads();
+    // Every load of a step is unconditional (past the last chunk they repeat
+    // it, and the last step's staging write goes to the idle tile), so the
+    // compiler's load counters agree on every path through the loop and no
+    // wait covers loads issued after the ones it needs.
+    auto kc = [&](int c) __attribute__((always_inline)) { return kof(c < nk ? c : nk - 1); };
+    auto step = [&](auto set, int it) __attribute__((always_inline)) {
+      constexpr int S = decltype(set)::value;
+      if constexpr (EARLY_B) bload(std::integral_constant<int, 1 - S>{}, kc(it + 1));
+      mfma_chunk(set, it & 1);
+      if constexpr (!EARLY_B) bload(set, kc(it + 1));
+      lstore((it + 1) & 1);
+      gload(kc(it + 2));
+      __syncthreads();
+    };
+    if constexpr (EARLY_B) {
+      int it = 0;
+      for (; it + 1 < nk; it += 2) {
+        step(S0{}, it);
+        step(S1{}, it + 1);
+      }
+      if (it < nk) step(S0{}, it);
+    } else {
+      for (int it = 0; it < nk; ++it) step(S0{}, it);
+    }
+  } else {
+    for (int it = 0; it < nk; ++it) {
+      lstore(0);
+      __syncthreads();
+      if (it + 1 < nk) gload(kof(it + 1));
+      mfma_chunk(S0{}, 0);
+      if (it + 1 < nk) bload(S0{}, kof(it + 1));
+      __syncthreads();
+    }
   }
 
   if constexpr (F16) {  // remove the power-of-two row and column scales
@@ -593,18 +642,22 @@ bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
 }
 
 // Default (UPT, NBW) per unit height and precision, from tile sweeps on
-// MI355X (K = 256-320, 256 output columns).  BF16-split modes at 48-row units
-// take four units per tile (the weight fragments of a chunk are reused over
-// 12 row blocks), FP32 and FP16X3 two (four units leave FP16X3 at two waves
-// per SIMD).
+// MI355X (K = 256-320, 256 output columns, jq_bench_gemm).  At 48-row units
+// the 16-bit modes take four units per tile (the weight fragments of a chunk
+// are reused over 12 row blocks), FP32 two.  FP16X3 at 128-row units takes
+// 256-column tiles when the layer is wider than 128 columns (the operand tile
+// is staged and split once for twice the columns).
 void default_cfg(int R, int prec, bool wide, int* upt, int* nbw) {
-  (void)wide;
   *nbw = 2;
   switch (R) {
     case 16: *upt = 4; break;
     case 32: *upt = 2; break;
-    case 48: *upt = (prec == PREC_F32 || prec == PREC_FP16X3) ? 2 : 4; break;
+    case 48: *upt = prec == PREC_F32 ? 2 : 4; break;
     case 64: *upt = 2; break;
+    case 128:
+      *upt = 1;
+      if (prec == PREC_FP16X3 && wide) *nbw = 4;
+      break;
     default: *upt = 1; break;
   }
 }
