@@ -255,3 +255,23 @@ def test_width_adaptation_rule():
     pm = np.full(4, 0.6)
     w, pm, c = ham.adapt_width(0.1, pm, 7, 0.6, adapt_frequency=4)
     assert w == pytest.approx(0.11)
+
+
+@pytest.mark.parametrize("name", ["n2", "ring42", "h60"])
+def test_equilibrated_fixture_energies_reproduce(name):
+    # The equilibrated-walker fixtures (tests/golden/make_equilibrated.py)
+    # carry FP64 oracle energies; recompute one walker so a stale or damaged
+    # fixture fails here rather than shifting the reported accuracy.
+    import os
+
+    from oracle import hamiltonian as ham
+
+    path = os.path.join(os.path.dirname(__file__), "golden", f"equilibrated_{name}.npz")
+    f = np.load(path)
+    sysm, spec, *_ = S.workload(name)
+    p = P.cast_params(P.init_params(spec, seed=0), np.float64)
+    x = f["x"][:1].astype(np.float64)
+    ref = ham.local_energy(spec.as_oracle(), p, x, sysm.atoms.astype(np.float64), sysm.charges.astype(np.float64))
+    np.testing.assert_allclose(ref["e_loc"], f["e_loc"][:1], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(ref["logpsi"], f["logpsi"][:1], rtol=1e-11, atol=1e-11)
+    assert f["x"].shape[0] == f["e_loc"].shape[0] == f["mh_accept"].shape[0]
