@@ -30,6 +30,16 @@
 // [g*K/4, (g+1)*K/4), so one 16-byte LDS read per operand feeds four MFMA
 // steps (V is staged transposed, [d][j], to make its K index j contiguous).
 // The tiles of a wave run as interleaved, independent MFMA chains.
+//
+// H3 (the FP16X3 arithmetic mode): Q and K rows are held in LDS as two FP16
+// planes, x s = h + l with h = fp16(x s), l = fp16(x s - h), where s is a
+// power of two per row that puts the row's largest |entry| in [2^13, 2^14)
+// (the row's 16 staging lanes agree on it by an xor-shuffle maximum; rows that
+// are all zero take s = 1).  The q.k products then run on
+// v_mfma_f32_16x16x32_f16 as hh + hl + lh (operand error 2^-22), one
+// accumulator per product, each multiplied by the inverse row scales before
+// they are combined.  The FP16 planes take the LDS the FP32 rows took; V, the
+// weights and every elementwise step stay FP32.
 
 #include "kernels.h"
 
@@ -38,15 +48,24 @@ namespace {
 constexpr int ATT_THREADS = 512;
 constexpr int ATT_WAVES = ATT_THREADS / 64;
 
-template <int NT, int DH>
+template <int NT, int DH, bool H3>
 struct AttnLayout {
   static constexpr int NP = 16 * NT;   // padded electrons
-  static constexpr int SQ = DH + 4;    // Q, K rows [i][d]
+  static constexpr int SQ = DH + 4;    // FP32 Q, K rows [i][d] (floats)
+  static constexpr int SQH = DH + 8;   // H3: FP16 Q, K rows (halves; 16-byte reads of 8 rows on distinct banks)
+  static constexpr int QKF = H3 ? NP * SQH : NP * SQ;  // floats of one Q or K array (H3: both planes)
   static constexpr int SJ = NP + 4;    // [i][j] arrays and transposed V [d][j]
   // Q_0, K_0, two row buffers of (Q_c, K_c); V_0 and two row buffers of V_c;
-  // a, Ja_c, J_1; the partial row sums
-  static constexpr int FLOATS = 6 * NP * SQ + 3 * DH * SJ + 3 * NP * SJ + NT * NP;
+  // a, Ja_c, J_1; the partial row sums; H3: inverse row scales of the six
+  // Q / K arrays
+  static constexpr int FLOATS = 6 * QKF + 3 * DH * SJ + 3 * NP * SJ + NT * NP + (H3 ? 6 * NP : 0);
 };
+
+// Maximum over the `n` consecutive lanes of an aligned lane group (n = 8, 16).
+__device__ inline float group_max(float v, int n) {
+  for (int o = 1; o < n; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
 
 // Sum over the 16 lanes of each DPP row (lanes 16r .. 16r+15), result in every
 // lane of the row: quad_perm [1,0,3,2] and [2,3,0,1] (xor 1, xor 2), then
@@ -59,10 +78,10 @@ __device__ inline float row16_sum(float v) {
   return v;
 }
 
-template <int NT, int DH>
+template <int NT, int DH, bool H3>
 __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
-  using L = AttnLayout<NT, DH>;
-  constexpr int NP = L::NP, SQ = L::SQ, SJ = L::SJ;
+  using L = AttnLayout<NT, DH, H3>;
+  constexpr int NP = L::NP, SQ = L::SQ, SJ = L::SJ, SQH = L::SQH, QKF = L::QKF;
   constexpr int DT = DH / 16;                               // output column tiles
   constexpr int JT = NT * NT, OT = NT * DT;                  // score / output tiles
   constexpr int JTW = (JT + ATT_WAVES - 1) / ATT_WAVES, OTW = (OT + ATT_WAVES - 1) / ATT_WAVES;
@@ -72,16 +91,20 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
   constexpr int KQ = DH / 4, KA = NP / 4;                    // K range per lane group (q.k / a.v)
 
   extern __shared__ float sm[];
-  float* Q0 = sm;             // Q_0, then K_0 at + NP*SQ
-  float* K0 = Q0 + NP * SQ;
-  float* Qb = K0 + NP * SQ;   // row c in buffer c % 2: Q_c at Qb + 2*NP*SQ*(c%2), K_c after it
-  float* V0 = Qb + 4 * NP * SQ;  // V_0 transposed [d][j]
+  float* Q0 = sm;             // Q_0, then K_0 at + QKF
+  float* K0 = Q0 + QKF;
+  float* Qb = K0 + QKF;       // row c in buffer c % 2: Q_c at Qb + 2*QKF*(c%2), K_c after it
+  float* V0 = Qb + 4 * QKF;   // V_0 transposed [d][j]
   float* Vb = V0 + DH * SJ;   // V_c transposed, buffer c % 2 at Vb + DH*SJ*(c%2)
   float* As = Vb + 2 * DH * SJ;  // attention weights a[i][j]
   float* Jt = As + NP * SJ;   // Ja_c (then S3, then lap a)
   float* J1 = Jt + NP * SJ;   // J_1
   float* Mp = J1 + NP * SJ;   // per column tile partial row sums sum_j a J_c  [Qt][i]
   float* RJ = Q0;             // after the tangent loop: rj [i][j]
+  // H3: inverse row scales, one NP array per Q / K array, in the arrays' order
+  // (Q_0, K_0, Q buffer 0, K buffer 0, Q buffer 1, K buffer 1)
+  float* IS = Mp + NT * NP;
+  auto isc = [&](const float* arr) __attribute__((always_inline)) { return IS + (int)((arr - Q0) / QKF) * NP; };
 
   const int N = a.N, RP = a.Rp, R = a.R;
   const int z = blockIdx.x, w = z / a.H, h = z % a.H;
@@ -127,6 +150,43 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
     for (int p = 0; p < PER; ++p) {
       int m, i, q;
       item(tid + ATT_THREADS * p, m, i, q);
+      if constexpr (H3) {
+        // the F4 lanes of a Q / K row form an aligned lane group: they agree
+        // on the row's power-of-two scale, then write their four entries as
+        // FP16 high and low parts (every lane takes part in the shuffles)
+        const float4 x = pf[p];
+        float mx = (m < 2 && i < N) ? fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))) : 0.f;
+        mx = group_max(mx, F4);
+        if (m < 2 && i < N) {
+          float sc = 1.f, si = 1.f;
+          if (mx > 0.f && mx <= 3.0e38f) {
+            int e;
+            (void)frexpf(mx, &e);
+            int sh = 14 - e;
+            sh = sh < -60 ? -60 : (sh > 60 ? 60 : sh);
+            sc = ldexpf(1.f, sh);
+            si = ldexpf(1.f, -sh);
+          }
+          const float xs[4] = {x.x * sc, x.y * sc, x.z * sc, x.w * sc};
+          f16 hi[4], lo[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            hi[u] = (f16)xs[u];
+            lo[u] = (f16)(xs[u] - (float)hi[u]);
+          }
+          f16* base = (f16*)(dq + m * QKF);  // plane 0 rows, then plane 1 rows
+          *(uint2*)(base + i * SQH + 4 * q) = *(const uint2*)hi;
+          *(uint2*)(base + NP * SQH + i * SQH + 4 * q) = *(const uint2*)lo;
+          if (q == 0) isc(dq + m * QKF)[i] = si;
+        }
+        if (m == 2 && i < N) {
+          dv[(4 * q + 0) * SJ + i] = x.x;
+          dv[(4 * q + 1) * SJ + i] = x.y;
+          dv[(4 * q + 2) * SJ + i] = x.z;
+          dv[(4 * q + 3) * SJ + i] = x.w;
+        }
+        continue;
+      }
       if (m < 2 && i < N) {
         *(float4*)(dq + (m * NP + i) * SQ + 4 * q) = pf[p];
       } else if (m == 2 && i < N) {
@@ -146,6 +206,19 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, y.y, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.z, y.z, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.w, y.w, acc, 0, 0, 0);
+  };
+  // H3: FP16 (high, low) fragments of row `row` of a Q / K array, k = 32 ks + 8 lg .. + 7
+  struct Frag {
+    f16x8 h, l;
+  };
+  auto rdh = [&](const float* arr, int row, int ks) __attribute__((always_inline)) {
+    const f16* b = (const f16*)arr + row * SQH + 32 * ks + 8 * lg;
+    return Frag{*(const f16x8*)b, *(const f16x8*)(b + NP * SQH)};
+  };
+  auto mf3 = [&](f32x4& acc, const Frag& x, const Frag& y) __attribute__((always_inline)) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(x.l, y.h, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(x.h, y.l, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(x.h, y.h, acc, 0, 0, 0);
   };
   auto put_tile = [&](float* D, const f32x4& acc, int P, int Qt) __attribute__((always_inline)) {
 #pragma unroll
@@ -183,21 +256,42 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
     f32x4 acc[JTW];
 #pragma unroll
     for (int t = 0; t < JTW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (H3) {
 #pragma unroll
-    for (int g = 0; g < KQ / 4; ++g)
+      for (int ks = 0; ks < DH / 32; ++ks)
+#pragma unroll
+        for (int t = 0; t < JTW; ++t) {
+          const int tile = wave + ATT_WAVES * t;
+          if (tile < JT) mf3(acc[t], rdh(Q0, 16 * (tile / NT) + lr, ks), rdh(K0, 16 * (tile % NT) + lr, ks));
+        }
+      const float* iq = isc(Q0);
+      const float* ik = isc(K0);
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
         const int tile = wave + ATT_WAVES * t;
-        if (tile < JT)
-          mfma4(acc[t], rd4(Q0, SQ, 16 * (tile / NT) + lr, lg * KQ, g), rd4(K0, SQ, 16 * (tile % NT) + lr, lg * KQ, g));
+        if (tile < JT) {
+          const float kj = ik[16 * (tile % NT) + lr];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[t][v] *= iq[16 * (tile / NT) + 4 * lg + v] * kj;
+        }
       }
+    } else {
+#pragma unroll
+      for (int g = 0; g < KQ / 4; ++g)
+#pragma unroll
+        for (int t = 0; t < JTW; ++t) {
+          const int tile = wave + ATT_WAVES * t;
+          if (tile < JT)
+            mfma4(acc[t], rd4(Q0, SQ, 16 * (tile / NT) + lr, lg * KQ, g), rd4(K0, SQ, 16 * (tile % NT) + lr, lg * KQ, g));
+        }
+    }
 #pragma unroll
     for (int t = 0; t < JTW; ++t) {
       const int tile = wave + ATT_WAVES * t;
       if (tile < JT) put_tile(As, acc[t], tile / NT, tile % NT);
     }
   }
-  commit(Qb + 2 * NP * SQ, Vb + DH * SJ);  // row 1 into buffer 1
+  commit(Qb + 2 * QKF, Vb + DH * SJ);  // row 1 into buffer 1
   fetch(2);
   __syncthreads();
 #pragma unroll
@@ -244,28 +338,66 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
   // LDS before the second.
   for (int c = 1; c < R; ++c) {
     const int b = c & 1;
-    const float* Qc = Qb + 2 * NP * SQ * b;
-    const float* Kc = Qc + NP * SQ;
+    const float* Qc = Qb + 2 * QKF * b;
+    const float* Kc = Qc + QKF;
     const float* Vc = Vb + DH * SJ * b;
     // J_c = Q_c K_0^T + Q_0 K_c^T  and  S3 += Q_c K_c^T
     f32x4 acc[JTW];
 #pragma unroll
     for (int t = 0; t < JTW; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const bool cross = c >= ROW_J0;
+    if constexpr (H3) {
+      // one accumulator per product (each carries its own row scales)
+      f32x4 a2[JTW], a3[JTW];
 #pragma unroll
-    for (int g = 0; g < KQ / 4; ++g)
+      for (int t = 0; t < JTW; ++t) a2[t] = a3[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < DH / 32; ++ks)
+#pragma unroll
+        for (int t = 0; t < JTW; ++t) {
+          const int tile = wave + ATT_WAVES * t;
+          if (tile < JT) {
+            const int P = tile / NT, Qt = tile % NT;
+            const Frag qc = rdh(Qc, 16 * P + lr, ks), q0 = rdh(Q0, 16 * P + lr, ks);
+            const Frag k0 = rdh(K0, 16 * Qt + lr, ks), kc = rdh(Kc, 16 * Qt + lr, ks);
+            mf3(acc[t], qc, k0);
+            mf3(a2[t], q0, kc);
+            if (cross) mf3(a3[t], qc, kc);
+          }
+        }
+      const float *iqc = isc(Qc), *ikc = isc(Kc), *iq0 = isc(Q0), *ik0 = isc(K0);
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
         const int tile = wave + ATT_WAVES * t;
         if (tile < JT) {
           const int P = tile / NT, Qt = tile % NT;
-          const float4 qc = rd4(Qc, SQ, 16 * P + lr, lg * KQ, g), q0 = rd4(Q0, SQ, 16 * P + lr, lg * KQ, g);
-          const float4 k0 = rd4(K0, SQ, 16 * Qt + lr, lg * KQ, g), kc = rd4(Kc, SQ, 16 * Qt + lr, lg * KQ, g);
-          mfma4(acc[t], qc, k0);
-          mfma4(acc[t], q0, kc);
-          if (cross) mfma4(s3[t], qc, kc);
+          const int j = 16 * Qt + lr;
+          const float sk0 = ik0[j], skc = ikc[j];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = 16 * P + 4 * lg + v;
+            const float sqc = iqc[i];
+            acc[t][v] = acc[t][v] * (sqc * sk0) + a2[t][v] * (iq0[i] * skc);
+            if (cross) s3[t][v] += a3[t][v] * (sqc * skc);
+          }
         }
       }
+    } else {
+#pragma unroll
+      for (int g = 0; g < KQ / 4; ++g)
+#pragma unroll
+        for (int t = 0; t < JTW; ++t) {
+          const int tile = wave + ATT_WAVES * t;
+          if (tile < JT) {
+            const int P = tile / NT, Qt = tile % NT;
+            const float4 qc = rd4(Qc, SQ, 16 * P + lr, lg * KQ, g), q0 = rd4(Q0, SQ, 16 * P + lr, lg * KQ, g);
+            const float4 k0 = rd4(K0, SQ, 16 * Qt + lr, lg * KQ, g), kc = rd4(Kc, SQ, 16 * Qt + lr, lg * KQ, g);
+            mfma4(acc[t], qc, k0);
+            mfma4(acc[t], q0, kc);
+            if (cross) mfma4(s3[t], qc, kc);
+          }
+        }
+    }
     if (c >= ROW_J0) {
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
@@ -311,7 +443,7 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
       }
     }
     if (c + 1 < R) {
-      commit(Qb + 2 * NP * SQ * (b ^ 1), Vb + DH * SJ * (b ^ 1));
+      commit(Qb + 2 * QKF * (b ^ 1), Vb + DH * SJ * (b ^ 1));
       fetch(c + 2);
     }
     __syncthreads();
@@ -395,17 +527,30 @@ __global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
   }
 }
 
-template <int NT, int DH>
+template <int NT, int DH, bool H3>
 int launch_one(const AttnArgs& a, hipStream_t s) {
-  const size_t smem = (size_t)AttnLayout<NT, DH>::FLOATS * 4;
+  const size_t smem = (size_t)AttnLayout<NT, DH, H3>::FLOATS * 4;
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void*)attn_lap_kernel<NT, DH>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void*)attn_lap_kernel<NT, DH, H3>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)smem);
     attr = true;
   }
-  hipLaunchKernelGGL((attn_lap_kernel<NT, DH>), dim3(a.nz), dim3(ATT_THREADS), smem, s, a);
+  hipLaunchKernelGGL((attn_lap_kernel<NT, DH, H3>), dim3(a.nz), dim3(ATT_THREADS), smem, s, a);
   return 0;
+}
+
+template <bool H3>
+int launch_mode(const AttnArgs& a, hipStream_t s) {
+  const int nt = (a.N + 15) / 16;
+  if (a.dh == 64) {
+    if (nt == 1) return launch_one<1, 64, H3>(a, s);
+    if (nt == 2) return launch_one<2, 64, H3>(a, s);
+    return launch_one<3, 64, H3>(a, s);
+  }
+  if (nt == 1) return launch_one<1, 32, H3>(a, s);
+  if (nt == 2) return launch_one<2, 32, H3>(a, s);
+  return launch_one<3, 32, H3>(a, s);
 }
 
 }  // namespace
@@ -414,13 +559,5 @@ bool attn_lap_supported(int N, int dh) { return N >= 1 && N <= 48 && (dh == 32 |
 
 int launch_attn_lap(const AttnArgs& a, hipStream_t s) {
   if (!attn_lap_supported(a.N, a.dh) || a.nz <= 0 || a.R > a.Rp || a.R < ROW_J0 || a.ldo % 4) return -1;
-  const int nt = (a.N + 15) / 16;
-  if (a.dh == 64) {
-    if (nt == 1) return launch_one<1, 64>(a, s);
-    if (nt == 2) return launch_one<2, 64>(a, s);
-    return launch_one<3, 64>(a, s);
-  }
-  if (nt == 1) return launch_one<1, 32>(a, s);
-  if (nt == 2) return launch_one<2, 32>(a, s);
-  return launch_one<3, 32>(a, s);
+  return a.h3 ? launch_mode<true>(a, s) : launch_mode<false>(a, s);
 }

@@ -157,6 +157,7 @@ struct AttnArgs {
   const float* QKV;
   float* O; int ldo;
   int N, Rp, R, H, dh, nz;
+  int h3;  // 1: the q.k products in the FP16X3 split (PREC_FP16X3), else FP32
 };
 bool attn_lap_supported(int N, int dh);
 int launch_attn_lap(const AttnArgs& a, hipStream_t s);

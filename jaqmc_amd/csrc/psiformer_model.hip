@@ -281,6 +281,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       aa.H = H;
       aa.dh = dh;
       aa.nz = Bc * H;
+      aa.h3 = prec == PREC_FP16X3 ? 1 : 0;
       rc |= launch_attn_lap(aa, s);
     } else {
       ProfScope ps(lap ? PROF_ATTN_LAP : -1, s);
