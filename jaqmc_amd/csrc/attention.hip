@@ -24,7 +24,7 @@
 // writes O_c straight to the attention output; two workgroup barriers per
 // row.  The score tensors never leave LDS.
 //
-// Eight waves share each product's 16x16 output tiles.  MFMA 16x16x4 (f32):
+// The workgroup's waves (att_threads) share each product's 16x16 output tiles.  MFMA 16x16x4 (f32):
 // lane (r = lane%16, g = lane/16) supplies A[r][k] and B[k][r]; result
 // element v is row 4g + v, column r.  Lane group g owns the contiguous K range
 // [g*K/4, (g+1)*K/4), so one 16-byte LDS read per operand feeds four MFMA
@@ -45,8 +45,11 @@
 
 namespace {
 
-constexpr int ATT_THREADS = 512;
-constexpr int ATT_WAVES = ATT_THREADS / 64;
+// Threads per workgroup: 16 waves (four per SIMD, to hide the LDS and
+// barrier latency of the row loop) where the registers fit 128 per lane, 12
+// waves for the FP16X3 variant (its three accumulators and FP16 fragment pairs
+// need about 150).
+constexpr int att_threads(bool h3) { return h3 ? 768 : 1024; }
 
 template <int NT, int DH, bool H3>
 struct AttnLayout {
@@ -79,7 +82,9 @@ __device__ inline float row16_sum(float v) {
 }
 
 template <int NT, int DH, bool H3>
-__global__ __launch_bounds__(ATT_THREADS) void attn_lap_kernel(AttnArgs a) {
+__global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
+  constexpr int ATT_THREADS = att_threads(H3);
+  constexpr int ATT_WAVES = ATT_THREADS / 64;
   using L = AttnLayout<NT, DH, H3>;
   constexpr int NP = L::NP, SQ = L::SQ, SJ = L::SJ, SQH = L::SQH, QKF = L::QKF;
   constexpr int DT = DH / 16;                               // output column tiles
@@ -536,7 +541,7 @@ int launch_one(const AttnArgs& a, hipStream_t s) {
                               (int)smem);
     attr = true;
   }
-  hipLaunchKernelGGL((attn_lap_kernel<NT, DH, H3>), dim3(a.nz), dim3(ATT_THREADS), smem, s, a);
+  hipLaunchKernelGGL((attn_lap_kernel<NT, DH, H3>), dim3(a.nz), dim3(att_threads(H3)), smem, s, a);
   return 0;
 }
 
