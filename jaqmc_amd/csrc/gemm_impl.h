@@ -276,22 +276,18 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         if ((unit + 1) * (16 * MBU) <= a.rows) {
           const long w = a.wmaj ? unit / (a.n_up + a.n_dn) : walker_of_unit((int)unit, a.Bc, a.n_up, a.n_dn);
           const float* gu = a.G + w * (16 * MBU) * a.ldg;
+          // loaded straight into the accumulators, all at once (no
+          // temporaries); columns >= ncols read column ncols - 1 and are
+          // never stored.  FP16X3 scales them after the first chunk's loads
+          // are issued (below).
 #pragma unroll
-          for (int m = 0; m < MBU; ++m) {
+          for (int m = 0; m < MBU; ++m)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const float* gr = gu + (long)(m * 16 + 4 * lg + v) * a.ldg;
-              // FP16X3: the accumulators carry the row and column scales
-              const float gs = F16 ? rsc[u * 16 * MBU + m * 16 + 4 * lg + v] : 1.f;
 #pragma unroll
-              for (int nb = 0; nb < NBW; ++nb) {
-                const float g = gr[ncl[nb]];
-                acc[u * MBU + m][nb][v] = cin[nb] ? (F16 ? g * gs * csf[nb] : g) : 0.f;
-              }
+              for (int nb = 0; nb < NBW; ++nb) acc[u * MBU + m][nb][v] = gr[ncl[nb]];
             }
-            // at most four row blocks of loads in flight (register pressure)
-            if (m % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-          }
         }
       }
     }
@@ -363,16 +359,35 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   };
   // EARLY_B: the next chunk's B fragments are requested into the other set
   // before this chunk's products, so no fragment is ever copied and no wait
-  // covers loads issued after them.  Otherwise (three planes or wide tiles,
-  // where two sets do not fit the register budget) one set is used and the
-  // next chunk's fragments are requested after the last product (L2 hits that
-  // arrive during the staging step).
-  constexpr bool EARLY_B = F32 || NS * NBW <= 4;
+  // covers loads issued after them.  Otherwise (three planes, or tiles of
+  // four column blocks per wave in FP32, where two sets do not fit the
+  // register budget) one set is used and the next chunk's fragments are
+  // requested after the last product (L2 hits that arrive during the staging
+  // step).
+  constexpr bool EARLY_B = F32 ? NBW <= 2 : NS * NBW <= 4;
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
 
   gload(kof(0));
   bload(S0{}, kof(0));
+  if constexpr (LAP) {
+    if (a.G) {
+      // The broadcast loads and the first chunk's are in flight together;
+      // wait for all of them here (vmcnt 0), so the K loop starts with no
+      // load outstanding from before it and its waits stay exact.
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      if constexpr (F16) {  // the accumulators carry the row and column scales
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float gs = rsc[mb * 16 + 4 * lg + v];
+#pragma unroll
+            for (int nb = 0; nb < NBW; ++nb) acc[mb][nb][v] *= gs * csf[nb];
+          }
+      }
+    }
+  }
   if constexpr (DB) {
     // chunk it is multiplied from tile it&1 while chunk it+1 is staged into
     // the other tile (its loads were issued one chunk earlier); the barrier
@@ -622,10 +637,12 @@ void launch_one(const GemmArgs& a, hipStream_t s) {
 }
 
 
+// Tile shapes that exist per unit height; FP32 has no four-block (256-column)
+// tiles: with the FP32 weight fragments they exceed 256 VGPRs and spill.
 template <int MBU, int PREC>
 bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
 #define JQ_TRY(U, NB)                                                           \
-  if constexpr (MBU * U <= 12 && MBU * U * NB <= 36) {                          \
+  if constexpr (MBU * U <= 12 && MBU * U * NB <= 36 && !(PREC == PREC_F32 && NB == 4)) { \
     if (upt == U && nbw == NB) {                                                \
       launch_one<MBU, U, NB, PREC, true>(a, s);                                 \
       return true;                                                              \
@@ -673,9 +690,13 @@ int dispatch(const GemmArgs& a, hipStream_t s) {
     // so the last round is not a mostly idle tail.
     const bool wide_tile = g_gemm_variant > 0 ? (g_gemm_variant % 10) == 4 : false;
     const long tiles128 = (long)ceil_div(a.rows, 128) * ceil_div(a.ncols, 128) * (a.nbatch > 1 ? a.nbatch : 1);
-    if (wide_tile)
-      launch_one<8, 1, 4, PREC, false>(a, s);
-    else if (tiles128 < 2L * 3 * 256)
+    if constexpr (PREC != PREC_F32) {  // no 256-column FP32 tiles (they spill)
+      if (wide_tile) {
+        launch_one<8, 1, 4, PREC, false>(a, s);
+        return 0;
+      }
+    }
+    if (tiles128 < 2L * 3 * 256)
       launch_one<4, 1, 2, PREC, false>(a, s);
     else
       launch_one<8, 1, 2, PREC, false>(a, s);
