@@ -274,8 +274,20 @@ def main():
     alg_bytes = RF.layer_bytes(spec, system.atoms.shape[0]) * B * args.steps
     traffic, traffic_src = measured_traffic(args.workload, args.mixed)
     avg_s = tot_ms * 1e-3 / max(n, 1)
-    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": traffic,
+    # the binding roof: the larger of the minimum MFMA time (algorithmic
+    # FLOPs x MFMAs per product at the mode's dense peak) and the minimum HBM
+    # time (algorithmic bytes at 8 TB/s)
+    t_mfma = fl["gemm_layer_lap"] * B * args.steps * mult / peak
+    t_hbm = alg_bytes / RF.PEAKS["hbm"]
+    if t_hbm >= t_mfma and n:
+        bw = alg_bytes / (tot_ms * 1e-3)
+        head = {"bound": "hbm", "achieved": bw / 1e9, "peak": RF.PEAKS["hbm"] / 1e9, "unit": "GB/s",
+                "frac": bw / RF.PEAKS["hbm"]}
+    else:
+        head = {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / peak}
+    roof = {**head, "traffic": traffic,
+            "algorithmic_tflops": achieved / 1e12, "mfma_peak_tflops": peak / 1e12, "mfma_frac": achieved / peak,
             "traffic_unit": "bytes/launch", "traffic_source": traffic_src,
             "mfma_per_product": mult, "issued_mfma_frac": achieved * mult / peak,
             "alg_bytes_per_launch": alg_bytes / max(n, 1),

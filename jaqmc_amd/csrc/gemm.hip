@@ -44,6 +44,7 @@ int launch_gemm(const GemmArgs& a, int prec, bool y_bf16, hipStream_t s) {
   if (y_bf16) return -1;
   if (a.K % 32 != 0 || a.ldx % 4 != 0 || a.ldw % 4 != 0) return -1;
   if (a.R > 1 && a.rows % a.R != 0) return -1;
+  if (a.rgrp > 0 && (a.R > 1 || a.Res || a.G || a.hs_H > 0 || a.nbatch > 1 || a.lrp > 0)) return -1;
   if (a.rows == 0) return 0;
   switch (prec) {
     case PREC_F32: return gemm_dispatch_f32(a, s);

@@ -132,6 +132,17 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   const long row0 = (long)rt * BM;
   const int col0 = cb * BN;
   const float* X = (const float*)a.X + (long)blockIdx.z * a.sxb;
+  // element offset of row gr in X (ld) or Y (ldy): row groups of rgrp rows
+  // at group strides sxg / syg when rgrp > 0 (plain-row launches only)
+  auto roff = [&](long gr, long ld, long sg) __attribute__((always_inline)) {
+    if constexpr (!LAP) {
+      if (a.rgrp > 0) {
+        const long g = gr / a.rgrp;
+        return g * sg + (gr - g * a.rgrp) * ld;
+      }
+    }
+    return gr * ld;
+  };
   const float* W = (const float*)a.Wt;
   // FP16X3 weights: two FP16 planes after the three BF16 planes, then the
   // inverse column scales and the column scales (powers of two)
@@ -143,15 +154,20 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // compiler then counts outstanding loads exactly).  Such rows and columns
   // only reach accumulators that are never stored: rows are independent,
   // Laplacian-mode units are whole, and output columns >= ncols are skipped.
+  // the element offsets of the thread's AV staging rows (and column), fixed
+  // for the tile
+  long xo[AV];
+#pragma unroll
+  for (int v = 0; v < AV; ++v) {
+    int idx = tid + v * 256;
+    if (idx >= BM * 8) idx -= BM * 8;  // duplicate of an in-tile load (not staged)
+    const int r = idx >> 3, q = idx & 7;
+    const long gr = row0 + r < a.rows ? row0 + r : (long)a.rows - 1;
+    xo[v] = roff(gr, a.ldx, a.sxg) + q * 4;
+  }
   auto gload = [&](int k0) __attribute__((always_inline)) {
 #pragma clang loop unroll(full)
-    for (int v = 0; v < AV; ++v) {
-      int idx = tid + v * 256;
-      if (idx >= BM * 8) idx -= BM * 8;  // duplicate of an in-tile load (not staged)
-      const int r = idx >> 3, q = idx & 7;
-      const long gr = row0 + r < a.rows ? row0 + r : (long)a.rows - 1;
-      pa[v] = *(const f32x4*)(X + gr * a.ldx + k0 + q * 4);
-    }
+    for (int v = 0; v < AV; ++v) pa[v] = *(const f32x4*)(X + xo[v] + k0);
   };
   auto bload = [&](auto set, int k0) __attribute__((always_inline)) {
     constexpr int S = decltype(set)::value;
@@ -223,7 +239,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       if (r < 0 || r >= BM || gr >= a.rows) continue;
       float mx = 0.f;
       for (int k = lane * 4; k < a.K; k += 256) {
-        const float4 v = *(const float4*)(X + gr * a.ldx + k);
+        const float4 v = *(const float4*)(X + roff(gr, a.ldx, a.sxg) + k);
         mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
       }
       mx = wave_max(mx);
@@ -551,7 +567,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       if (grow >= a.rows) continue;
       if (!nok) {  // padding columns [ncols, zpad) are written as zeros
         if (n < a.zpad && a.hs_H == 0) {
-          float* dz = Y + grow * a.ldy + n;
+          float* dz = Y + roff(grow, a.ldy, a.syg) + n;
 #pragma unroll
           for (int q = 0; q < 4; ++q)
             if (n + q < a.zpad) dz[q] = 0.f;
@@ -595,7 +611,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         }
         dst = Y + ((((long)w * 3 * a.hs_H) * a.hs_N + i) * RU + rr) * (long)a.hs_dh + hcol;
       } else {
-        dst = Y + grow * a.ldy + n;
+        dst = Y + roff(grow, a.ldy, a.syg) + n;
         if (Res) {
           const float* rp = Res + grow * a.ldres + n;
           if (!tail) {

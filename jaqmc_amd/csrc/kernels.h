@@ -46,6 +46,8 @@ struct GemmArgs {
                      // of each unit by powers of two before the FP16 split (set by the caller
                      // for R = 1 launches on Laplacian-mode data; R > 1 implies lrp = R)
   int zpad;          // > ncols: columns [ncols, zpad) of Y are written as zeros
+  int rgrp;          // > 0 (plain rows only): row r is row r % rgrp of group r / rgrp, at
+  long sxg, syg;     // X + (r / rgrp) * sxg + (r % rgrp) * ldx and Y likewise with syg, ldy
 };
 
 // Batched GEMM on activation operands (both FP32 in HBM, split on load for the

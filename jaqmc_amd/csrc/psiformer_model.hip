@@ -410,9 +410,16 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       go.n_up = n_up;
       go.n_dn = n_dn;
     }
-    go.nbatch = Bc;
-    go.sxb = (long)N * RP * D;
-    go.syb = (long)N * RP * ldphi;
+    if (lap) {  // a walker's channel rows are whole 128-row tiles: batch over walkers
+      go.nbatch = Bc;
+      go.sxb = (long)N * RP * D;
+      go.syb = (long)N * RP * ldphi;
+    } else {  // value rows: one launch over the walkers' channel rows as row groups
+      go.rows = Bc * ns;
+      go.rgrp = ns;
+      go.sxg = (long)N * D;
+      go.syg = (long)N * ldphi;
+    }
     ProfScope ps(lap ? PROF_GEMM_ORB_LAP : -1, s);
     if (launch_gemm(go, prec, false, s)) return jq_fail(JQ_EINVAL, "orbital GEMM launch rejected");
   }
