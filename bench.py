@@ -200,7 +200,7 @@ def cpu_baseline(spec, system, params, x, width):
     return x.shape[0] / t, t, threads, phys, model
 
 
-CPU_SAMPLE = {"li": 1024, "n2": 512, "ring42": 6, "h60": 2}
+CPU_SAMPLE = {"li": 1024, "n2": 512, "ring42": 12, "h60": 4}
 
 
 def main():

@@ -178,6 +178,37 @@ __global__ __launch_bounds__(256) void layernorm_lap_kernel(int Rp, int R, const
   }
 }
 
+// Value rows only (Rp = 1, the Metropolis forward pass): y = scale (x - mu)
+// s + bias with s = 1 / sqrt(mean((x - mu)^2) + eps) -- the rule above without
+// tangent rows.  One wavefront per row (D / 64 features per lane), four rows
+// per workgroup; mean and variance are wavefront sums.
+template <int D>
+__global__ __launch_bounds__(256) void layernorm_fwd_kernel(long rows, const float* __restrict__ x,
+                                                            float* __restrict__ y, const float* __restrict__ scale,
+                                                            const float* __restrict__ bias, float eps) {
+  constexpr int PL = D / 64;  // features per lane
+  const long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const int f0 = lane * PL;
+  float v[PL];
+#pragma unroll
+  for (int e = 0; e < PL; ++e) v[e] = x[row * D + f0 + e];
+  float m = 0.f;
+#pragma unroll
+  for (int e = 0; e < PL; ++e) m += v[e];
+  m = wave_sum(m) * (1.f / D);
+  float sq = 0.f;
+#pragma unroll
+  for (int e = 0; e < PL; ++e) {
+    v[e] -= m;
+    sq += v[e] * v[e];
+  }
+  const float sv = rsqrtf(wave_sum(sq) * (1.f / D) + eps);
+#pragma unroll
+  for (int e = 0; e < PL; ++e) y[row * D + f0 + e] = scale[f0 + e] * (v[e] * sv) + bias[f0 + e];
+}
+
 // The same rule with the unit's Rp x D tile read from HBM once into LDS, used
 // when the tile fits.  Rows are D + 4 floats apart, so every LDS access is a
 // 16-byte access: the row-per-thread statistics passes read four features at
@@ -465,6 +496,16 @@ bool launch_ln_lds(int units, int Rp, int R, const float* x, float* y, const flo
 int launch_layernorm_lap(int units, int Rp, int R, int D, const float* x, float* y, const float* scale,
                          const float* bias, float eps, hipStream_t s) {
   if (Rp > 256) return -1;
+  if (Rp == 1 && units > 0) {  // value rows only
+    const unsigned g = (unsigned)((units + 3) / 4);
+    switch (D) {
+      case 64: hipLaunchKernelGGL(layernorm_fwd_kernel<64>, dim3(g), dim3(256), 0, s, (long)units, x, y, scale, bias, eps); return 0;
+      case 128: hipLaunchKernelGGL(layernorm_fwd_kernel<128>, dim3(g), dim3(256), 0, s, (long)units, x, y, scale, bias, eps); return 0;
+      case 256: hipLaunchKernelGGL(layernorm_fwd_kernel<256>, dim3(g), dim3(256), 0, s, (long)units, x, y, scale, bias, eps); return 0;
+      case 512: hipLaunchKernelGGL(layernorm_fwd_kernel<512>, dim3(g), dim3(256), 0, s, (long)units, x, y, scale, bias, eps); return 0;
+    }
+    return -1;
+  }
   if (Rp > 1 && units > 0) {  // Laplacian mode: tile resident in LDS when it fits
     bool ok = false;
     if (D == 64) ok = launch_ln_lds<64>(units, Rp, R, x, y, scale, bias, eps, s);
