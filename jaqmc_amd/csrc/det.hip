@@ -10,8 +10,9 @@
 //   lap logdet      = tr(A^-1 L) - sum_c tr((A^-1 J_c)^2)
 // with B_c = A^-1 J_c on FP64 MFMA.  Three kernels: det_small_kernel
 // (N <= 16, one wavefront per determinant), det_mfma_kernel (16 < N <= 64,
-// Laplacian mode, one workgroup per determinant) and det_fwd_kernel
-// (16 < N <= 64, log|det| and sign only, one wavefront per determinant).
+// Laplacian mode, one workgroup per determinant) and det_fwd_reg_kernel
+// (16 < N <= 64, log|det| and sign only, one wavefront per determinant with
+// the matrix in registers).
 
 #include <algorithm>
 #include <type_traits>
@@ -39,7 +40,7 @@ struct LogAbsProduct {
 
 // Orbital-matrix element (value, and optionally Jacobian row c or Laplacian).
 __device__ inline const float* env_ptr(const FermiDims& d, const float* env, int w, int i, int k, int o) {
-  return env + ((((long)w * d.N + i) * d.K + k) * d.N + o) * 5;
+  return env + ((((long)w * d.N + i) * d.K + k) * d.N + o) * env_stride(d);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,85 +365,113 @@ __global__ __launch_bounds__(256, (NP <= 48 ? 2 : 1)) void det_mfma_kernel(Fermi
 }
 
 // ---------------------------------------------------------------------------
-// Forward mode (log|det| and sign only), 16 < N <= 64: one wavefront per
-// determinant, two per workgroup, no workgroup barriers.  LU with partial
-// pivoting on the N x N block in the wave's LDS slice (no inverse is formed):
-// lane (ri, cj) = (lane / 8, lane % 8) updates rows p+1+ri+8a, columns
-// p+1+cj+8b of the trailing block, with its pivot-row entries in registers.
+// Forward mode, 16 < N <= 64: log|det| and sign with the matrix in registers.
+// One wavefront per determinant, four per workgroup, no LDS: lane i holds row
+// i of A (NP >= N doubles, NP a multiple of 8, zero-padded; lanes i >= N hold
+// no row).  Elimination without row exchanges: the pivot of column p is the
+// unused row with the largest |A[i][p]| (compared on the FP32 rounding of the
+// magnitude with its low 6 bits replaced by the row, which picks the smaller
+// row on ties and a row within 2^-17 of the largest otherwise; non-finite
+// entries rank below every finite one, so a walker with a NaN still gets a
+// valid pivot row), its row is read from that lane with readlane, and every
+// other unused row subtracts its multiple of it from the columns after p.
+// det = sign(sigma) * prod of pivots with sigma(p) = pivot row of column p; the
+// parity of sigma is the number of unused rows below the pivot, summed over
+// the steps.  The column loop is unrolled at compile time (column p is
+// register p), so the update is NP - 1 - p fused multiply-adds per lane with
+// the pivot-row entry in scalar registers.
 // ---------------------------------------------------------------------------
-template <int NP>
-__global__ __launch_bounds__(128) void det_fwd_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
-                                                      const float* __restrict__ env, double* __restrict__ out) {
-  constexpr int S = NP + 1;  // odd row stride (doubles)
-  constexpr int NB8 = NP / 8;
-  extern __shared__ double smf[];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // determinants 2cb, 2cb + 1 of walker w; a walker's blocks share an XCD
-  int w, cb;
-  xcd_tile(blockIdx.x, (d.K + 1) / 2, &w, &cb);
-  const int k = 2 * cb + wv;
-  if (w >= d.Bc || k >= d.K) return;  // the wave's own determinant; nothing is shared across waves
-  const long det = (long)w * d.K + k;
-  double* M = smf + wv * NP * S;
-  const int N = d.N;
-#pragma unroll 8
-  for (int e = lane; e < N * N; e += 64) {
-    const int i = e / N, o = e - i * N;
-    M[i * S + o] = (double)Phi[(long)unit_idx(d, w, i) * d.Rp * ldphi + (long)k * N + o] *
-                   (double)env_ptr(d, env, w, i, k, o)[0];
-  }
-  wave_sync();
+__device__ inline double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Maximum of a 32-bit key over the wavefront: DPP within rows of 16, then the
+// four row results through scalar registers.
+__device__ inline unsigned wave_max_u32(unsigned k) {
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  k = max(k, (unsigned)__builtin_amdgcn_mov_dpp((int)k, 0x140, 0xF, 0xF, false));  // row_mirror
+  const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)k, 0);
+  const unsigned r1 = (unsigned)__builtin_amdgcn_readlane((int)k, 16);
+  const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)k, 32);
+  const unsigned r3 = (unsigned)__builtin_amdgcn_readlane((int)k, 48);
+  return max(max(r0, r1), max(r2, r3));
+}
+
+struct FwdLU {
+  unsigned long long used;  // bit i: row i is a pivot row already (or holds no row)
   LogAbsProduct logdet;
-  int sgn = 1;
-  const int ri = lane >> 3, cj = lane & 7;
-  for (int p = 0; p < N; ++p) {
-    // largest |M[i][p]| over rows i >= p, smallest i on ties; lanes without a
-    // row offer none (index NP), so non-finite entries still yield a valid,
-    // wave-uniform pivot row
-    double best = (p + lane < N) ? fabs(M[(p + lane) * S + p]) : -1.0;
-    int bi = (p + lane < N) ? p + lane : NP;
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ob = __shfl_xor(best, o);
-      const int oi = __shfl_xor(bi, o);
-      if (oi != NP && (bi == NP || ob > best || (ob == best && oi < bi))) {
-        best = ob;
-        bi = oi;
-      }
-    }
-    bi = __builtin_amdgcn_readlane(bi, 0);
-    if (bi >= N) bi = p;
-    if (bi != p) {
-      if (lane < N) {
-        const double t = M[p * S + lane];
-        M[p * S + lane] = M[bi * S + lane];
-        M[bi * S + lane] = t;
-      }
-      sgn = -sgn;
-    }
-    wave_sync();
-    const double pv = M[p * S + p];
-    logdet.mul(pv);
-    if (pv < 0) sgn = -sgn;
-    const double rp = 1.0 / pv;
-    double prow[NB8];
+  int neg, parity;
+};
+
+template <int P, int NP>
+__device__ __forceinline__ void fwd_lu_steps(double (&m)[NP], int N, int lane, FwdLU& st) {
+  if constexpr (P < NP) {
+    if (P < N) {
+      const double x = m[P];
+      const double ax = fabs(x);
+      const bool open = !((st.used >> lane) & 1ull);
+      const unsigned tie = 63u - (unsigned)lane;
+      const unsigned key =
+          !open ? 0u : (ax <= 1.7976931348623157e308 ? ((__float_as_uint((float)ax) & ~63u) + 64u) | tie : tie);
+      // key 0 decodes to lane 63: it wins only when every key is 0, i.e. the
+      // one open row is a non-finite row 63
+      const int piv = 63 - (int)(wave_max_u32(key) & 63u);
+      const unsigned long long below = ~st.used & ((1ull << piv) - 1ull);
+      st.parity ^= __builtin_popcountll(below) & 1;
+      st.used |= 1ull << piv;
+      const double pv = readlane_f64(x, piv);
+      st.logdet.mul(pv);
+      st.neg ^= pv < 0.0 ? 1 : 0;
+      const double l = (open && lane != piv) ? x / pv : 0.0;
+      // pivot-row entries read four at a time ahead of their multiply-adds, so
+      // the scalar reads do not wait on each other
 #pragma unroll
-    for (int b = 0; b < NB8; ++b) {
-      const int j = p + 1 + cj + 8 * b;
-      prow[b] = j < N ? M[p * S + j] : 0.0;
-    }
-    for (int i = p + 1 + ri; i < N; i += 8) {
-      const double l = M[i * S + p] * rp;
+      for (int j0 = P + 1; j0 < NP; j0 += 4) {
+        double r[4];
 #pragma unroll
-      for (int b = 0; b < NB8; ++b) {
-        const int j = p + 1 + cj + 8 * b;
-        if (j < N) M[i * S + j] -= l * prow[b];
+        for (int q = 0; q < 4; ++q)
+          if (j0 + q < NP) r[q] = readlane_f64(m[j0 + q], piv);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (j0 + q < NP) m[j0 + q] = fma(-l, r[q], m[j0 + q]);
       }
     }
-    wave_sync();
+    fwd_lu_steps<P + 1, NP>(m, N, lane, st);
   }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void det_fwd_reg_kernel(FermiDims d, const float* __restrict__ Phi, long ldphi,
+                                                          const float* __restrict__ env, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int w, cb;
+  xcd_tile(blockIdx.x, (d.K + 3) / 4, &w, &cb);
+  const int k = 4 * cb + wv;
+  if (w >= d.Bc || k >= d.K) return;  // the wave's own determinant
+  const long det = (long)w * d.K + k;
+  const int N = d.N;
+  double m[NP];
+  {
+    const int i = lane < N ? lane : 0;
+    const float* ph = Phi + (long)unit_idx(d, w, i) * d.Rp * ldphi + (long)k * N;
+    const float* ep = env_ptr(d, env, w, i, k, 0);
+#pragma unroll
+    for (int j = 0; j < NP; ++j)
+      m[j] = (lane < N && j < N) ? (double)ph[j] * (double)ep[j] : 0.0;  // value-only envelope (stride 1)
+  }
+  FwdLU st;
+  st.used = N >= 64 ? 0ull : ~((1ull << N) - 1ull);
+  st.neg = 0;
+  st.parity = 0;
+  fwd_lu_steps<0, NP>(m, N, lane, st);
   if (lane == 0) {
-    out[det * 3] = logdet.log_abs();
-    out[det * 3 + 1] = (double)sgn;
+    out[det * 3] = st.logdet.log_abs();
+    out[det * 3 + 1] = (st.neg ^ st.parity) ? -1.0 : 1.0;
     out[det * 3 + 2] = 0.0;
   }
 }
@@ -503,7 +532,7 @@ __global__ __launch_bounds__(256, 4) void det_small_kernel(FermiDims d, const fl
     if (in) {
       const int u = unit_idx(d, w, i);
       base[t] = (long)u * Rp * ldphi + k * N + o;
-      const float* ep = env + ((((long)w * N + i) * d.K + k) * N + o) * 5;
+      const float* ep = env_ptr(d, env, w, i, k, o);
       phi0[t] = Phi[base[t]];
       e0[t] = ep[0];
       if (lapmode) {
@@ -678,7 +707,7 @@ __global__ __launch_bounds__(256, 4) void det_small_kernel(FermiDims d, const fl
   for (int t = 0; t < 4; ++t) {
     if (base[t] < 0) continue;
     const int e = lane + 64 * t, i = e >> 4, o = e & 15;
-    const float* ep = env + ((((long)w * N + i) * d.K + k) * N + o) * 5;
+    const float* ep = env + ((((long)w * N + i) * d.K + k) * N + o) * 5;  // Laplacian mode: 5 floats
     double L = (double)Phi[base[t] + ROW_LAP * ldphi] * e0[t] + (double)phi0[t] * ep[1];
     for (int ax = 0; ax < 3; ++ax) L += 2.0 * (double)Phi[base[t] + (ROW_J0 + 3 * i + ax) * ldphi] * ej[t][ax];
     part += S.M[o * 32 + 16 + i] * L;
@@ -933,7 +962,7 @@ __global__ void orbital_export_kernel(FermiDims d, const float* __restrict__ Phi
     const int i = (int)(wi % d.N), w = (int)(wi / d.N);
     const int k = ko / d.N, o = ko % d.N;
     const float phi = Phi[(long)unit_idx(d, w, i) * d.Rp * ldphi + ko];
-    out[(((long)w * d.K + k) * d.N + i) * d.N + o] = phi * env[(wi * KN + ko) * 5];
+    out[(((long)w * d.K + k) * d.N + i) * d.N + o] = phi * env[(wi * KN + ko) * env_stride(d)];
   }
 }
 
@@ -976,18 +1005,14 @@ int launch_det(const FermiDims& d, const float* Phi, long ldphi, const float* en
     }
   }
   if (d.Rp <= 1 && d.N <= 64) {
-    const int np = round_up(d.N, 16);
-    const size_t smem = (size_t)2 * np * (np + 1) * sizeof(double);
-    static bool attr_f = false;
-    if (!attr_f) {
-      (void)hipFuncSetAttribute((const void*)det_fwd_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024);
-      attr_f = true;
-    }
-    const dim3 grid((unsigned)xcd_grid(d.Bc, (d.K + 1) / 2));
-    switch (np) {
-      case 32: hipLaunchKernelGGL(det_fwd_kernel<32>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out); return 0;
-      case 48: hipLaunchKernelGGL(det_fwd_kernel<48>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out); return 0;
-      case 64: hipLaunchKernelGGL(det_fwd_kernel<64>, grid, dim3(128), smem, s, d, Phi, ldphi, env, out); return 0;
+    const dim3 grid4((unsigned)xcd_grid(d.Bc, (d.K + 3) / 4));
+    switch (round_up(d.N, 8)) {
+      case 24: hipLaunchKernelGGL(det_fwd_reg_kernel<24>, grid4, dim3(256), 0, s, d, Phi, ldphi, env, out); return 0;
+      case 32: hipLaunchKernelGGL(det_fwd_reg_kernel<32>, grid4, dim3(256), 0, s, d, Phi, ldphi, env, out); return 0;
+      case 40: hipLaunchKernelGGL(det_fwd_reg_kernel<40>, grid4, dim3(256), 0, s, d, Phi, ldphi, env, out); return 0;
+      case 48: hipLaunchKernelGGL(det_fwd_reg_kernel<48>, grid4, dim3(256), 0, s, d, Phi, ldphi, env, out); return 0;
+      case 56: hipLaunchKernelGGL(det_fwd_reg_kernel<56>, grid4, dim3(256), 0, s, d, Phi, ldphi, env, out); return 0;
+      case 64: hipLaunchKernelGGL(det_fwd_reg_kernel<64>, grid4, dim3(256), 0, s, d, Phi, ldphi, env, out); return 0;
     }
   }
   return -1;  // N > 64

@@ -395,10 +395,12 @@ __global__ void spin_means_kernel(FermiDims d, const T* __restrict__ X, long ldx
 // Envelope (wavefunction/output/envelope.py:111-140), per electron i of spin
 // channel s, det k, orbital o:  e = sum_I pi[o,I,k] exp(-a r_iI), a = |sigma| or
 // sigma.  Gradient wrt x_i: sum_I -a f (x_i - R_I)/r; Laplacian: sum_I f (a^2 - 2a/r).
-// Output env[w][i][k*N + o][5] = (value, Laplacian, d/dx, d/dy, d/dz).
+// Output env[w][i][k*N + o][5] = (value, Laplacian, d/dx, d/dy, d/dz) in
+// Laplacian mode, env[w][i][k*N + o] = value in forward mode.
 // One workgroup per (walker, electron): the electron-nucleus displacements are
 // computed once into LDS, and the threads walk (k, o) with the parameters laid
 // out [I][k][o] (coalesced across the threads for each nucleus).
+template <bool LAP>
 __global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float* __restrict__ elec,
                                                        const float* __restrict__ atoms, const float* __restrict__ pi0,
                                                        const float* __restrict__ sg0, const float* __restrict__ pi1,
@@ -407,11 +409,12 @@ __global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float*
   extern __shared__ float geo[];  // [M][4]: dx, dy, dz, r; then [M]: 1 / r
   const int wi = blockIdx.x, i = wi % d.N;
   const int KN = d.K * d.N;
-  float* out = env + (long)wi * KN * 5;
+  constexpr int ES = LAP ? 5 : 1;  // env_stride
+  float* out = env + (long)wi * KN * ES;
   if (kind == 2) {  // null envelope
     for (int ko = threadIdx.x; ko < KN; ko += blockDim.x) {
-      out[ko * 5] = 1.f;
-      out[ko * 5 + 1] = out[ko * 5 + 2] = out[ko * 5 + 3] = out[ko * 5 + 4] = 0.f;
+      out[ko * ES] = 1.f;
+      if constexpr (LAP) out[ko * 5 + 1] = out[ko * 5 + 2] = out[ko * 5 + 3] = out[ko * 5 + 4] = 0.f;
     }
     return;
   }
@@ -437,17 +440,21 @@ __global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float*
       const float a = kind == 1 ? fabsf(sg[(long)I * KN + ko]) : sg[(long)I * KN + ko];
       const float f = pi[(long)I * KN + ko] * expf(-a * g.w);
       v += f;
-      l += f * a * (a - 2.f * ir);
-      const float gs = -a * f * ir;
-      gx += gs * g.x;
-      gy += gs * g.y;
-      gz += gs * g.z;
+      if constexpr (LAP) {
+        l += f * a * (a - 2.f * ir);
+        const float gs = -a * f * ir;
+        gx += gs * g.x;
+        gy += gs * g.y;
+        gz += gs * g.z;
+      }
     }
-    out[ko * 5] = v;
-    out[ko * 5 + 1] = l;
-    out[ko * 5 + 2] = gx;
-    out[ko * 5 + 3] = gy;
-    out[ko * 5 + 4] = gz;
+    out[ko * ES] = v;
+    if constexpr (LAP) {
+      out[ko * 5 + 1] = l;
+      out[ko * 5 + 2] = gx;
+      out[ko * 5 + 3] = gy;
+      out[ko * 5 + 4] = gz;
+    }
   }
 }
 
@@ -593,7 +600,11 @@ int launch_envelope(const FermiDims& d, const float* elec, const float* atoms, c
   if ((long)d.Bc * d.N <= 0) return 0;
   const size_t smem = (size_t)5 * d.M * sizeof(float);
   if (smem > 64 * 1024) return -1;
-  hipLaunchKernelGGL(envelope_kernel, dim3((unsigned)(d.Bc * d.N)), 256, smem, s, d, elec, atoms, pi[0], sigma[0],
-                     d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
+  if (env_stride(d) == 5)
+    hipLaunchKernelGGL(envelope_kernel<true>, dim3((unsigned)(d.Bc * d.N)), 256, smem, s, d, elec, atoms, pi[0],
+                       sigma[0], d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
+  else
+    hipLaunchKernelGGL(envelope_kernel<false>, dim3((unsigned)(d.Bc * d.N)), 256, smem, s, d, elec, atoms, pi[0],
+                       sigma[0], d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
   return 0;
 }

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void det_vjp_kernel(FermiDims d, const float* 
     double v;
     if (j < N)
       v = (double)Phi[(long)unit_idx(d, w, i) * ldphi + k * N + j] *
-          (double)env[(((long)w * N + i) * KN + k * N + j) * 5];
+          (double)env[(((long)w * N + i) * KN + k * N + j) * env_stride(d)];
     else
       v = (j - N == i) ? 1.0 : 0.0;
     M[i][j] = v;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void det_vjp_kernel(FermiDims d, const float* 
     const double da = scale * M[o][N + i];
     const long u = unit_idx(d, w, i);
     const float ph = Phi[u * ldphi + k * N + o];
-    const float ev = env[(((long)w * N + i) * KN + k * N + o) * 5];
+    const float ev = env[(((long)w * N + i) * KN + k * N + o) * env_stride(d)];
     dPhi[u * lddphi + k * N + o] = (float)(da * ev);
     denv[((long)w * N + i) * KN + k * N + o] = (float)(da * ph);
   }

@@ -81,6 +81,10 @@ struct FermiDims {
 __host__ __device__ inline int unit_idx(const FermiDims& d, int w, int i) {
   return d.wmaj ? w * d.N + i : unit_of(w, i, d.Bc, d.n_up, d.n_dn);
 }
+// Floats per envelope element in env[walker, i, k*N + o, :]: value,
+// Laplacian and gradient in Laplacian mode (Rp > 1), the value alone in
+// forward mode.
+__host__ __device__ inline int env_stride(const FermiDims& d) { return d.Rp > 1 ? 5 : 1; }
 // Layer-0 one-electron input (ae features) into X[:, :, 0:4M] and the pair
 // stream input H2 (RP2 rows per pair: 8 in Laplacian mode, 1 in forward mode).
 int launch_ferminet_features(const FermiDims& d, const float* elec, const float* atoms,
@@ -95,7 +99,7 @@ int launch_pair_means(const FermiDims& d, const float* H2, int f2, void* X, long
 // Spin-channel means of X[:, :, 0:f1] -> g[walker, r, ch*f1 + f]  (ldg row stride).
 int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void* g, long ldg,
                       bool bf16io, hipStream_t s);
-// Envelope value / Laplacian / gradient: env[walker, i, k*N + o, 5].
+// Envelope value / Laplacian / gradient: env[walker, i, k*N + o, env_stride(d)].
 int launch_envelope(const FermiDims& d, const float* elec, const float* atoms, const float* const* pi,
                     const float* const* sigma, int env_kind, float* env, hipStream_t s);
 

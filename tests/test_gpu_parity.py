@@ -371,6 +371,36 @@ def test_coincident_electrons_are_data(cuda, case):
     assert (not np.isfinite(v)) or v < lp0[2].item() - 5.0, (v, lp0[2].item())
 
 
+@pytest.mark.parametrize("n", [20, 31, 33, 40, 47, 52, 57, 63])
+def test_forward_logdet_every_padded_size(cuda, n):
+    """log|psi| and sign on the forward (Metropolis) path for hydrogen chains
+    whose electron counts reach every padded size of the register LU kernel
+    (24 ... 64 in steps of 8), one determinant so log|psi| and sign are those
+    of det A itself; against the FP64 oracle (jnp.linalg.slogdet,
+    wavefunction/output/logdet.py:65) within the log|det| perturbation bound."""
+    import torch
+
+    sysm = S.h_chain(n)
+    _, spec, *_ = S.workload("li")
+    spec.nspins = sysm.nspins
+    spec.n_atoms = n
+    spec.hidden_single, spec.hidden_double, spec.ndets = [32, 32], [8, 8], 1
+    p = P.init_params(spec, seed=5, bias_scale=0.1, env_jitter=0.2)
+    x = S.init_walkers(sysm, 6, seed=6)
+    p64 = P.cast_params(p, np.float64)
+    at = sysm.atoms.astype(np.float64)
+    sg_ref, lp_ref = owf.logpsi(spec.as_oracle(), p64, x.astype(np.float64), at, with_laplacian=False)
+    A = np.asarray(owf.orbital_matrices(spec.as_oracle(), p64, x.astype(np.float64), at))
+    kappa = np.linalg.cond(A).reshape(x.shape[0], -1).max(axis=1)
+    dm = _model(sysm, spec, p)
+    lp, sg = dm.logpsi(torch.from_numpy(x).to(cuda))
+    torch.cuda.synchronize()
+    d = np.abs(lp.cpu().numpy() - lp_ref.v)
+    assert np.all(d <= logpsi_bound(lp_ref.v, kappa, "fp32")), (d, kappa)
+    np.testing.assert_array_equal(sg.cpu().numpy(), np.asarray(sg_ref).reshape(-1))
+    dm.close()
+
+
 def test_psiformer_chunking_and_bf16x6(cuda):
     """Psiformer: chunked evaluation is bit-exact, and bf16x6 stays at FP32-level error."""
     import torch
