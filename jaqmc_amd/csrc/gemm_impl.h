@@ -217,6 +217,30 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     }
   };
 
+  // When the residual is a slice of this GEMM's own input (X[row][n] for
+  // output column n < ncols), the K loop starts after the residual columns
+  // and ends on them, so the epilogue's residual reads follow the row tile's
+  // most recent loads of those columns.  Every column block of a row tile
+  // walks K in the same order (they run back to back on one XCD and share
+  // the X panel's loads in its L2).
+  const int nk = a.K / KC;
+  const bool res_in_x = a.Res != nullptr && (const void*)a.Res == a.X && a.ldres == a.ldx && a.ncols <= a.K;
+  const int kst = res_in_x ? ((a.ncols + KC - 1) / KC) % nk : 0;
+  auto kof = [&](int it) __attribute__((always_inline)) {
+    const int c = it + kst;
+    return (c >= nk ? c - nk : c) * KC;
+  };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  // The first chunk's operands are requested before the row-scale scan, so
+  // their latency and the scan's overlap (not in tiles of 32 accumulator
+  // blocks, whose registers are all taken by then).
+  constexpr bool FIRST_EARLY = MB * NBW < 32;
+  if constexpr (FIRST_EARLY) {
+    gload(kof(0));
+    bload(S0{}, kof(0));
+  }
+
   // PREC_FP16X3 row scales: every row takes the fixed Jacobian scale, then
   // the value and Laplacian rows of every unit (row % period 0 or 1) are
   // scanned over K, one wave per row.  The candidate rows of the tile are
@@ -309,19 +333,6 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     }
   }
 
-  // When the residual is a slice of this GEMM's own input (X[row][n] for
-  // output column n < ncols), the K loop starts after the residual columns
-  // and ends on them, so the epilogue's residual reads follow the row tile's
-  // most recent loads of those columns.  Every column block of a row tile
-  // walks K in the same order (they run back to back on one XCD and share
-  // the X panel's loads in its L2).
-  const int nk = a.K / KC;
-  const bool res_in_x = a.Res != nullptr && (const void*)a.Res == a.X && a.ldres == a.ldx && a.ncols <= a.K;
-  const int kst = res_in_x ? ((a.ncols + KC - 1) / KC) % nk : 0;
-  auto kof = [&](int it) __attribute__((always_inline)) {
-    const int c = it + kst;
-    return (c >= nk ? c - nk : c) * KC;
-  };
   // products of one chunk from operand tile `buf` with B fragment set `set`
   auto mfma_chunk = [&](auto set, int buf) __attribute__((always_inline)) {
     constexpr int S = decltype(set)::value;
@@ -381,11 +392,11 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // requested after the last product (L2 hits that arrive during the staging
   // step).
   constexpr bool EARLY_B = F32 ? NBW <= 2 : NS * NBW <= 4;
-  using S0 = std::integral_constant<int, 0>;
-  using S1 = std::integral_constant<int, 1>;
 
-  gload(kof(0));
-  bload(S0{}, kof(0));
+  if constexpr (!FIRST_EARLY) {
+    gload(kof(0));
+    bload(S0{}, kof(0));
+  }
   if constexpr (LAP) {
     if (a.G) {
       // The broadcast loads and the first chunk's are in flight together;
