@@ -40,11 +40,21 @@ int gemm_unit_rows(int R) {
   return -1;
 }
 
+// The tile table gives 128-row units in FP16X3 one unit per tile and four
+// 16-column blocks per wave (256 columns): a workgroup then holds every
+// column of its unit.  The dispatch rejects an Yln launch of any other tile.
+bool gemm_ln_fusable(int R, int ncols, int prec) {
+  return R == 128 && ncols == 256 && prec == PREC_FP16X3 && g_gemm_variant == 0;
+}
+
 int launch_gemm(const GemmArgs& a, int prec, bool y_bf16, hipStream_t s) {
   if (y_bf16) return -1;
   if (a.K % 32 != 0 || a.ldx % 4 != 0 || a.ldw % 4 != 0) return -1;
   if (a.R > 1 && a.rows % a.R != 0) return -1;
   if (a.rgrp > 0 && (a.R > 1 || a.Res || a.G || a.hs_H > 0 || a.nbatch > 1 || a.lrp > 0)) return -1;
+  if (a.Yln && (a.R <= 1 || a.G || a.hs_H > 0 || a.nbatch > 1 || a.ldy != a.ncols || a.ln_R > a.R ||
+                a.ln_R < ROW_J0 || !a.ln_s || !a.ln_b || (a.Res && a.ldres % 4 != 0)))
+    return -1;
   if (a.rows == 0) return 0;
   switch (prec) {
     case PREC_F32: return gemm_dispatch_f32(a, s);

@@ -76,7 +76,10 @@ constexpr int gemm_min_waves() {
   return MB * NBW <= 12 ? 3 : 2;
 }
 
-template <int MBU, int UPT, int NBW, int PREC, bool LAP>
+// LNORM: the fused-LayerNorm variant (GemmArgs::Yln set; no head-split,
+// broadcast or padding outputs), a kernel of its own so that the plain
+// variant keeps its register allocation.
+template <int MBU, int UPT, int NBW, int PREC, bool LAP, bool LNORM = false>
 __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_kernel(const GemmArgs a) {
   constexpr int MB = MBU * UPT;
   constexpr int BM = 16 * MB;
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       cin[nb] = n < a.ncols;
       ncl[nb] = cin[nb] ? n : a.ncols - 1;
     }
-    if (a.G) {
+    if (!LNORM && a.G) {
 #pragma unroll
       for (int u = 0; u < UPT; ++u) {
         const long unit = row0 / (16 * MBU) + u;
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     bload(S0{}, kof(0));
   }
   if constexpr (LAP) {
-    if (a.G) {
+    if (!LNORM && a.G) {
       // The broadcast loads and the first chunk's are in flight together;
       // wait for all of them here (vmcnt 0), so the K loop starts with no
       // load outstanding from before it and its waits stay exact.
@@ -477,7 +480,14 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // two shuffles across the lane groups.  Stores are 64-byte row segments.
   constexpr int RU = LAP ? 16 * MBU : 1;
   float* Y = (float*)a.Y + (long)blockIdx.z * a.syb;
-  const float* Res = (const float*)a.Res;
+  // With the fused LayerNorm the residual goes into the accumulators first
+  // (pass 4 normalises the sum), one 16-row block at a time so that few loads
+  // are in flight, and the store below adds none.
+  constexpr bool LNF = LNORM;
+  static_assert(!LNORM || (LAP && UPT == 1), "the fused LayerNorm needs whole units per tile");
+  constexpr bool lnf = LNF;
+  const float* Res = lnf ? nullptr : (const float*)a.Res;
+
   int ncol[NBW];
   bool cok[NBW];
 #pragma unroll
@@ -529,6 +539,22 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         }
     }
   }
+  if constexpr (LNF) {
+    if (lnf && a.Res) {
+      asm volatile("" ::: "memory");  // the residual loads start after the passes above
+      const float* rs = (const float*)a.Res;
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float* rp = rs + (row0 + mb * 16 + 4 * lg + v) * a.ldres;
+#pragma unroll
+          for (int nb = 0; nb < NBW; ++nb) acc[mb][nb][v] = (acc[mb][nb][v] + rp[ncol[nb]]) * a.res_scale;
+        }
+        asm volatile("" ::: "memory");  // keeps the next block's loads after these
+      }
+    }
+  }
   // pass 3: residual and store (row-major, or head-split for Psiformer Q/K/V).
   // Each wave stages one 16-row block of its columns at a time in its own
   // slice of the (now free) operand arena and writes it back with 16-byte
@@ -545,13 +571,13 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   const int c4 = (lane % C4W) * 4, r_lane = lane / C4W;
   const int n = col0 + wcol + c4;
   const bool nok = n < a.ncols;
-  const bool tail = !(n + 4 <= a.ncols && vec_io);
+  const bool tail = LNORM ? false : !(n + 4 <= a.ncols && vec_io);
   // head-split destination: column part ((part*H + h)*N*RU)*dh + d is per lane,
   // the row part ((w*3*H*N + i)*RU + rr)*dh per row; Laplacian-mode units of the
   // tile are consecutive, so (w, i) of its first unit is computed once.
   long hcol = 0;
   int hw0 = 0, hi0 = 0;
-  if (a.hs_H > 0) {
+  if (!LNORM && a.hs_H > 0) {
     const int part = n / a.hs_D, hh = (n % a.hs_D) / a.hs_dh, dd = n % a.hs_dh;
     hcol = ((long)part * a.hs_H + hh) * a.hs_N * ((long)RU * a.hs_dh) + dd;
     if (LAP) {
@@ -576,7 +602,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       const int r = r_lane + RPI * it;
       const long grow = row0 + mb0 * 16 + r;
       if (grow >= a.rows) continue;
-      if (!nok) {  // padding columns [ncols, zpad) are written as zeros
+      if (!LNORM && !nok) {  // padding columns [ncols, zpad) are written as zeros
         if (n < a.zpad && a.hs_H == 0) {
           float* dz = Y + roff(grow, a.ldy, a.syg) + n;
 #pragma unroll
@@ -604,7 +630,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         }
       }
       float* dst;
-      if (a.hs_H > 0) {
+      if (!LNORM && a.hs_H > 0) {
         int w, i, rr;
         if (LAP) {
           const int u = (mb0 * 16 + r) / RU;  // unit of the tile
@@ -652,15 +678,160 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     }
     wave_sync();
   }
+
+  // pass 4 (fused LayerNorm): Yln = LN(output) by the forward-Laplacian rule
+  // of layernorm_lap_kernel.  The tile is one unit with every column; each
+  // wave holds 64*NBW/4 columns of all its rows, so a row statistic is a sum
+  // over the 16 lanes of a lane group (one DPP row), combined across the four
+  // waves in LDS by one thread per row; the Laplacian row's sum over the
+  // Jacobian rows is an in-thread sum plus two shuffles across the lane
+  // groups.  Unit row r is acc[r / 16][.][r % 4] of lane group (r / 4) % 4.
+  if constexpr (LNF) {
+    if (lnf) {
+      constexpr float invD = 1.f / (64 * NBW);
+      float* const part = (float*)arena;  // [4][RU] per-wave row partials
+      float* const rowv = part + 4 * RU;  // [RU] per-row result (mean, then js)
+      float* const scal = rowv + RU;      // [8] unit partials and scalars
+      static_assert((5 * RU + 8) * 4 <= ARENA, "LayerNorm reductions must fit the arena");
+      auto gsum = [&](float t) __attribute__((always_inline)) {
+        t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0xB1, 0xF, 0xF, false));
+        t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x4E, 0xF, 0xF, false));
+        t += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x141, 0xF, 0xF, false));
+        return t + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x140, 0xF, 0xF, false));
+      };
+      __syncthreads();  // every wave's staging slice is free
+      // row means; the accumulators become the centred rows d_r (the output
+      // itself was stored above)
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          float t = 0.f;
+#pragma unroll
+          for (int nb = 0; nb < NBW; ++nb) t += acc[mb][nb][v];
+          t = gsum(t);
+          if (lr == 0) part[wave * RU + mb * 16 + 4 * lg + v] = t;
+        }
+      __syncthreads();
+      if (tid < RU) rowv[tid] = ((part[tid] + part[RU + tid]) + (part[2 * RU + tid] + part[3 * RU + tid])) * invD;
+      __syncthreads();
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const float m = rowv[mb * 16 + 4 * lg + v];
+#pragma unroll
+          for (int nb = 0; nb < NBW; ++nb) acc[mb][nb][v] -= m;
+        }
+      // centred value row of the lane's columns (row 0 sits in lane group 0)
+      float d0[NBW];
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) d0[nb] = __shfl(acc[0][nb][0], lr);
+      // <d_r, d_0> per row; sum of |d_r|^2 over the Jacobian rows
+      float sq = 0.f;
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = mb * 16 + 4 * lg + v;
+          float t = 0.f, q = 0.f;
+#pragma unroll
+          for (int nb = 0; nb < NBW; ++nb) {
+            t += acc[mb][nb][v] * d0[nb];
+            q += acc[mb][nb][v] * acc[mb][nb][v];
+          }
+          sq += (r >= ROW_J0 && r < a.ln_R) ? q : 0.f;
+          t = gsum(t);
+          if (lr == 0) part[wave * RU + r] = t;
+        }
+      sq = wave_sum(sq);
+      if (lane == 0) scal[wave] = sq;
+      __syncthreads();
+      // unit scalars: variance v, its Laplacian, s = (v + eps)^-1/2 and its
+      // first two derivatives; per Jacobian row js_r = s' J_v,r with
+      // J_v,r = 2/D <d_r, d_0> (one thread per row), and sum_r J_v,r^2
+      auto dotr = [&](int r) __attribute__((always_inline)) {
+        return (part[r] + part[RU + r]) + (part[2 * RU + r] + part[3 * RU + r]);
+      };
+      const float sqt = (scal[0] + scal[1]) + (scal[2] + scal[3]);
+      const float lapv = 2.f * invD * (dotr(ROW_LAP) + sqt);
+      const float ve = dotr(ROW_VAL) * invD + a.ln_eps;
+      const float sv = rsqrtf(ve);
+      const float s1 = -0.5f * sv / ve;
+      const float s2 = 0.75f * sv / (ve * ve);
+      float jv2 = 0.f;
+      if (tid < RU) {
+        const float jv = (tid >= ROW_J0 && tid < a.ln_R) ? 2.f * invD * dotr(tid) : 0.f;
+        rowv[tid] = s1 * jv;
+        jv2 = jv * jv;
+      }
+      jv2 = wave_sum(jv2);
+      __syncthreads();  // every wave has read part[] and scal[0..3]
+      if (lane == 0) scal[4 + wave] = jv2;
+      __syncthreads();
+      const float laps = s1 * lapv + s2 * ((scal[4] + scal[5]) + (scal[6] + scal[7]));
+      float g[NBW], bb[NBW], cross[NBW];
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) {
+        g[nb] = a.ln_s[ncol[nb]];
+        bb[nb] = a.ln_b[ncol[nb]];
+        cross[nb] = 0.f;
+      }
+      // value and Jacobian rows; the Laplacian row waits for
+      // cross = sum_r d_r js_r over the Jacobian rows
+#pragma unroll
+      for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = mb * 16 + 4 * lg + v;
+          const float js = rowv[r];  // 0 outside the Jacobian rows
+          const bool jr = r >= ROW_J0 && r < a.ln_R;
+#pragma unroll
+          for (int nb = 0; nb < NBW; ++nb) {
+            const float d = acc[mb][nb][v];
+            cross[nb] += d * js;
+            if (r == ROW_VAL)
+              acc[mb][nb][v] = g[nb] * (d * sv) + bb[nb];
+            else if (r != ROW_LAP)
+              acc[mb][nb][v] = jr ? g[nb] * (d * sv + d0[nb] * js) : 0.f;
+          }
+        }
+#pragma unroll
+      for (int nb = 0; nb < NBW; ++nb) {
+        float t = cross[nb] + __shfl_xor(cross[nb], 16);
+        t += __shfl_xor(t, 32);
+        if (lg == 0) acc[0][nb][ROW_LAP] = g[nb] * (acc[0][nb][ROW_LAP] * sv + d0[nb] * laps + 2.f * t);
+      }
+      __syncthreads();  // the reductions are read before the staging slices reuse the arena
+#pragma unroll 1
+      for (int mb0 = 0; mb0 < MB; ++mb0) {
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb)
+          if (mb == mb0) {
+#pragma unroll
+            for (int nb = 0; nb < NBW; ++nb)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) stg[(4 * lg + v) * SW + nb * 16 + lr] = acc[mb][nb][v];
+          }
+        wave_sync();
+#pragma unroll
+        for (int it = 0; it < 16 / RPI; ++it) {
+          const int r = r_lane + RPI * it;
+          *(float4*)(a.Yln + (row0 + mb0 * 16 + r) * a.ncols + n) = *(const float4*)&stg[r * SW + c4];
+        }
+        wave_sync();
+      }
+    }
+  }
 }
 
-template <int MBU, int UPT, int NBW, int PREC, bool LAP>
+template <int MBU, int UPT, int NBW, int PREC, bool LAP, bool LNORM = false>
 void launch_one(const GemmArgs& a, hipStream_t s) {
   constexpr int BM = 16 * MBU * UPT, BN = 64 * NBW;
   const int ntiles = a.linear_tiles ? ceil_div(a.rows, BM) * ceil_div(a.ncols, BN)
                                     : xcd_grid(ceil_div(a.rows, BM), ceil_div(a.ncols, BN));
   dim3 grid(ntiles, 1, a.nbatch > 1 ? a.nbatch : 1);
-  hipLaunchKernelGGL((gemm_kernel<MBU, UPT, NBW, PREC, LAP>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((gemm_kernel<MBU, UPT, NBW, PREC, LAP, LNORM>), grid, dim3(256), 0, s, a);
 }
 
 
@@ -675,6 +846,15 @@ bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
       return true;                                                              \
     }                                                                           \
   }
+  // the fused-LayerNorm launches: FP16X3, one 128-row unit and 256 columns per tile
+  if constexpr (PREC == PREC_FP16X3 && MBU == 8) {
+    if (a.Yln) {
+      if (upt != 1 || nbw != 4) return false;
+      launch_one<MBU, 1, 4, PREC, true, true>(a, s);
+      return true;
+    }
+  }
+  if (a.Yln) return false;
   JQ_TRY(1, 2)
   JQ_TRY(1, 4)
   JQ_TRY(2, 2)
@@ -710,6 +890,7 @@ template <int PREC>
 int dispatch(const GemmArgs& a, hipStream_t s) {
   const bool wide = a.ncols > 128;
   if (a.R <= 1) {
+    if (a.Yln) return -1;
     // two 128-column blocks share their X panel through the XCD's L2, which
     // beats one 256-column block at a third of the occupancy.  Launches of
     // fewer than two rounds of 128-row tiles over the CUs (the Metropolis
@@ -735,6 +916,7 @@ int dispatch(const GemmArgs& a, hipStream_t s) {
     upt = g_gemm_variant / 10;
     nbw = g_gemm_variant % 10;
   }
+  if (a.Yln && !(upt == 1 && 64 * nbw == a.ncols)) return -1;  // LayerNorm needs whole rows per workgroup
   bool ok = false;
   switch (a.R) {
     case 16: ok = launch_lap<1, PREC>(a, upt, nbw, s); break;

@@ -48,7 +48,19 @@ struct GemmArgs {
   int zpad;          // > ncols: columns [ncols, zpad) of Y are written as zeros
   int rgrp;          // > 0 (plain rows only): row r is row r % rgrp of group r / rgrp, at
   long sxg, syg;     // X + (r / rgrp) * sxg + (r % rgrp) * ldx and Y likewise with syg, ldy
+  // LayerNorm of the output in Laplacian mode (the forward-Laplacian rule of
+  // layernorm_lap_kernel): Yln (ld ncols) = LN(Y) with scale ln_s, bias ln_b,
+  // rows [ln_R, R) of every unit zero.  Only where one workgroup holds whole
+  // units and every column (gemm_ln_fusable); null: none.
+  float* Yln;
+  const float* ln_s;
+  const float* ln_b;
+  float ln_eps;
+  int ln_R;
 };
+// Whether a Laplacian-mode GEMM of R rows per unit and ncols output columns
+// can produce the LayerNorm of its output in its epilogue (GemmArgs::Yln).
+bool gemm_ln_fusable(int R, int ncols, int prec);
 
 // Batched GEMM on activation operands (both FP32 in HBM, split on load for the
 // BF16 modes): for z = w*bh + h, Y_z (+)= X_z . op(W_z) where op(W) is W^T

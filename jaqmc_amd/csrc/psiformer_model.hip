@@ -205,8 +205,13 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
   float* A1 = S3 + (long)Bc * H * N * NpS;
   float* A2 = A1 + (long)Bc * H * N * RP * Np32;
 
+  // Pre-LN layers on the Laplacian path: where a GEMM's workgroup holds whole
+  // units and every feature, the LayerNorm that follows the residual update
+  // runs in that GEMM's epilogue (Yln) instead of as a kernel of its own.
+  const bool lnf = lap && ln_mode == JQ_LN_PRE && gemm_ln_fusable(RP, D, prec);
   auto gemm = [&](const float* X, long ldx, void* Wt, void* Wlo, long wplane, int K_, float* Y, long ldy, int ncols,
-                  const float* bias, int act, const float* Res, int zpad = 0) {
+                  const float* bias, int act, const float* Res, int zpad = 0, float* yln = nullptr,
+                  const float* lns = nullptr, const float* lnb = nullptr) {
     GemmArgs g{};
     g.X = X;
     g.ldx = ldx;
@@ -230,19 +235,28 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     g.n_dn = n_dn;
     g.wmaj = 1;
     g.zpad = zpad;
+    g.Yln = yln;
+    g.ln_s = lns;
+    g.ln_b = lnb;
+    g.ln_eps = 1e-5f;
+    g.ln_R = R;
     ProfScope ps(lap ? PROF_GEMM_LAYER_LAP : -1, s);
     return launch_gemm(g, prec, false, s);
   };
 
   int rc = launch_psi_features(fd, elec, atoms, T2, kin, (options & JQ_OPT_NO_RESCALE) ? 0 : 1, s);
-  rc |= gemm(T2, kin, Win, Win_lo, (long)D * kin, kin, x, D, D, bin, ACT_NONE, nullptr);
+  // ln1_in_t1: T1 already holds LN1(x) of the coming layer (fused above)
+  bool ln1_in_t1 = lnf && L > 0;
+  rc |= gemm(T2, kin, Win, Win_lo, (long)D * kin, kin, x, D, D, bin, ACT_NONE, nullptr, 0, ln1_in_t1 ? T1 : nullptr,
+             L > 0 ? pl[0].ln1_s : nullptr, L > 0 ? pl[0].ln1_b : nullptr);
   for (int l = 0; l < L && !rc; ++l) {
     const PsiLayer& ly = pl[l];
     const float* qin = x;  // attention input: LN1(x) pre-LN, x otherwise
     if (ln_mode == JQ_LN_PRE) {
-      rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln1_s, ly.ln1_b, 1e-5f, s);
+      if (!ln1_in_t1) rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln1_s, ly.ln1_b, 1e-5f, s);
       qin = T1;
     }
+    ln1_in_t1 = false;
     {
       GemmArgs g{};
       g.X = qin;
@@ -351,8 +365,9 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
         rc |= launch_bgemm(g6, prec, s);
       }
     }
-    // x += Wo O + bo
-    rc |= gemm(Om, DO, ly.Wo, ly.Wo_lo, (long)D * D, D, x, D, D, ly.bo, ACT_NONE, x);
+    // x += Wo O + bo  (and T1 = LN2(x) when fused)
+    rc |= gemm(Om, DO, ly.Wo, ly.Wo_lo, (long)D * D, D, x, D, D, ly.bo, ACT_NONE, x, 0, lnf ? T1 : nullptr,
+               ly.ln2_s, ly.ln2_b);
     if (ln_mode == JQ_LN_POST) {  // x = LN1(x + attention)
       rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln1_s, ly.ln1_b, 1e-5f, s);
       std::swap(x, T1);
@@ -360,7 +375,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     // MLP on LN2(x) (pre-LN) or x, residual into x
     float* cur = x;
     if (ln_mode == JQ_LN_PRE) {
-      rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln2_s, ly.ln2_b, 1e-5f, s);
+      if (!lnf) rc |= launch_layernorm_lap((int)units, RP, R, D, x, T1, ly.ln2_s, ly.ln2_b, 1e-5f, s);
       cur = T1;
     }
     int curw = D;
@@ -370,10 +385,17 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       const int ka = round_up(curw, 32);
       const bool last = j == nm - 1;
       float* dst = last ? x : (cur == T1 ? T2 : T1);
+      // the last one writes LN1(x) of the next layer into T1 when fused.  If
+      // its input is T1 itself (one MLP layer), the layouts agree and every
+      // tile holds whole units and all columns, so each tile overwrites only
+      // rows it has finished reading.
+      const bool fuse_next = lnf && last && l + 1 < L && (cur != T1 || ka == D);
       // hidden outputs: padding columns up to the next K multiple are zeroed
       // for the following GEMM
       rc |= gemm(cur, ka, ly.Wm[j], ly.Wm_lo[j], (long)outw * ka, ka, dst, last ? D : round_up(outw, 32), outw,
-                 ly.bm[j], ACT_TANH, last ? x : nullptr, last ? 0 : round_up(outw, 32));
+                 ly.bm[j], ACT_TANH, last ? x : nullptr, last ? 0 : round_up(outw, 32), fuse_next ? T1 : nullptr,
+                 fuse_next ? pl[l + 1].ln1_s : nullptr, fuse_next ? pl[l + 1].ln1_b : nullptr);
+      ln1_in_t1 = fuse_next;
       cur = dst;
       curw = outw;
     }
