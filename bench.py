@@ -57,13 +57,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(spec, system, B, width, params, x0, precision, steps, warmup, S_steps):
+def run_path(spec, system, B, width, params, x0, precision, steps, warmup, S_steps, chunk=0):
     import torch
 
     from jaqmc_amd import _lib
     from jaqmc_amd.device import DeviceModel, reduce_mean_var
 
-    dm = DeviceModel(spec, system.atoms, system.charges, params, precision=precision)
+    dm = DeviceModel(spec, system.atoms, system.charges, params, precision=precision, chunk_walkers=chunk)
     x = torch.from_numpy(x0).cuda()
     lp, _ = dm.logpsi(x)
     logp = 2.0 * lp
@@ -211,6 +211,8 @@ def main():
     ap.add_argument("--workload", default="ring42", choices=["li", "n2", "ring42", "h60"])
     ap.add_argument("--mixed", default="fp16x3", choices=["fp16x3", "bf16x6", "bf16x3", "bf16"])
     ap.add_argument("--cpu-sample", type=int, default=0, help="walkers in the CPU-baseline sample (0: per workload)")
+    ap.add_argument("--workspace-gb", type=float, default=128.0,
+                    help="device memory per model for walker-chunk workspace (sets the chunk size; 0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--serving-jobs", type=int, default=12, help="jobs in the serving sample (0: skip)")
@@ -236,6 +238,16 @@ def main():
     system, spec, B, S_steps, width = S.workload(args.workload)
     params = P.init_params(spec, seed=0)
     x0 = S.init_walkers(system, B, seed=0)
+    # walkers per workspace chunk: the batch in equal chunks within the
+    # workspace budget (HBM is large enough to hold the whole 42-electron
+    # batch's tangent rows at once; fewer chunks leave fewer partial rounds)
+    chunk = 0
+    if args.workspace_gb > 0:
+        from jaqmc_amd.device import chunk_for_budget
+
+        chunk = chunk_for_budget(spec, system.atoms, system.charges, params, args.mixed, B,
+                                 args.workspace_gb * (1 << 30))
+    log(f"walkers per workspace chunk: {chunk or 'library default'}")
     if args.traffic_probe:
         from jaqmc_amd.device import DeviceModel
 
@@ -243,7 +255,7 @@ def main():
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        dm = DeviceModel(spec, system.atoms, system.charges, params, precision=args.mixed)
+        dm = DeviceModel(spec, system.atoms, system.charges, params, precision=args.mixed, chunk_walkers=chunk)
         dm.local_energy(torch.from_numpy(x0).cuda())
         torch.cuda.synchronize()
         log("traffic probe done: copy 1 GiB, one local-energy pass")
@@ -253,7 +265,7 @@ def main():
 
     res = {}
     for prec in ("fp32", args.mixed):
-        res[prec] = run_path(spec, system, B, width, params, x0, prec, args.steps, args.warmup, S_steps)
+        res[prec] = run_path(spec, system, B, width, params, x0, prec, args.steps, args.warmup, S_steps, chunk)
         log(prec, {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res[prec].items() if k != "prof"})
         log("  prof (ms total, launches):", res[prec]["prof"])
 
@@ -271,7 +283,7 @@ def main():
     achieved = fl["gemm_layer_lap"] * B * args.steps / (tot_ms * 1e-3) if n else 0.0
     peak = RF.PEAKS[RF.MODE_PEAK[args.mixed]]
     mult = RF.MFMA_PER_PRODUCT[args.mixed]
-    alg_bytes = RF.layer_bytes(spec, system.atoms.shape[0]) * B * args.steps
+    alg_bytes = RF.layer_bytes(spec, system.atoms.shape[0], args.mixed) * B * args.steps
     traffic, traffic_src = measured_traffic(args.workload, args.mixed)
     avg_s = tot_ms * 1e-3 / max(n, 1)
     # the binding roof: the larger of the minimum MFMA time (algorithmic
@@ -344,7 +356,8 @@ def main():
         "dtype": f"{args.mixed}+fp64",
         "data": "synthetic (seeded random-init params, Gaussian walkers)",
         "config": {"workload": f"{names[args.workload]}: {system.name} {arch} K={spec.ndets}", "walkers": B,
-                   "electrons": spec.n_elec, "mh_width": width, "protocol_mh_steps": S_steps},
+                   "electrons": spec.n_elec, "mh_width": width, "protocol_mh_steps": S_steps,
+                   "walkers_per_chunk": chunk or None},
         "fp32_value": res["fp32"]["value"],
         "fp32_ms_per_step": res["fp32"]["ms_per_step"],
         "mixed_over_fp32": mixed["value"] / res["fp32"]["value"],

@@ -481,8 +481,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   constexpr int RU = LAP ? 16 * MBU : 1;
   float* Y = (float*)a.Y + (long)blockIdx.z * a.syb;
   // With the fused LayerNorm the residual goes into the accumulators first
-  // (pass 4 normalises the sum), one 16-row block at a time so that few loads
-  // are in flight, and the store below adds none.
+  // (pass 4 normalises the sum), and the store below adds none.
   constexpr bool LNF = LNORM;
   static_assert(!LNORM || (LAP && UPT == 1), "the fused LayerNorm needs whole units per tile");
   constexpr bool lnf = LNF;
@@ -541,7 +540,6 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   }
   if constexpr (LNF) {
     if (lnf && a.Res) {
-      asm volatile("" ::: "memory");  // the residual loads start after the passes above
       const float* rs = (const float*)a.Res;
 #pragma unroll
       for (int mb = 0; mb < MB; ++mb) {
@@ -551,7 +549,6 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
 #pragma unroll
           for (int nb = 0; nb < NBW; ++nb) acc[mb][nb][v] = (acc[mb][nb][v] + rp[ncol[nb]]) * a.res_scale;
         }
-        asm volatile("" ::: "memory");  // keeps the next block's loads after these
       }
     }
   }

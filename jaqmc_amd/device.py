@@ -195,6 +195,23 @@ def reduce_mean_var(x):
 CLIP_METHODS = {"mad": 0, "iqr": 1, "none": 2}
 
 
+def chunk_for_budget(spec: ModelSpec, atoms, charges, params, precision, walkers, budget_bytes):
+    """Walkers per workspace chunk for a batch of `walkers` whose model should
+    hold at most `budget_bytes` of device memory: the workspace is affine in
+    the chunk (probed with chunks of one and two walkers), and the batch is
+    split into equal chunks of at most the largest chunk that fits."""
+    ws = []
+    for c in (1, 2):
+        m = DeviceModel(spec, atoms, charges, params, precision=precision, chunk_walkers=c)
+        ws.append(m.workspace_bytes)
+        m.close()
+    per = max(1, ws[1] - ws[0])
+    fixed = ws[0] - per
+    cmax = max(1, (int(budget_bytes) - fixed) // per)
+    n = -(-int(walkers) // cmax)
+    return -(-int(walkers) // n)
+
+
 def loss_clip(x, method="mad", scale=5.0, workspace=None):
     """LossAndGrad's clipping on the GPU (FP64): returns (clipped, weights,
     stats) with weights = 2 (clipped - mean(clipped)) / n and stats =

@@ -111,10 +111,23 @@ def ferminet_layer_bytes(spec: ModelSpec, n_atoms: int):
     return tot
 
 
-def psiformer_layer_bytes(spec: ModelSpec, n_atoms: int):
+def ln_fused(spec: ModelSpec, precision: str) -> bool:
+    """Whether the Laplacian-mode LayerNorms of a pre-LN Psiformer run in the
+    layer GEMMs' epilogue: FP16X3, 3N + 2 rows per unit rounded to 128 and 256
+    features (the library's gemm_ln_fusable)."""
+    if spec.kind != "psiformer" or precision != "fp16x3" or spec.layer_norm_mode != "pre":
+        return False
+    R = 3 * spec.n_elec + 2
+    rp = next((o for o in (16, 32, 48, 64, 96, 128, 192) if R <= o), -1)
+    return rp == 128 and spec.num_heads * spec.heads_dim == 256
+
+
+def psiformer_layer_bytes(spec: ModelSpec, n_atoms: int, fused_ln: bool = False):
     """Per-walker minimum HBM bytes of the Psiformer Laplacian-mode projections:
     Q/K/V (input, 3D output), output projection and last MLP layer (input,
-    residual stream read, residual stream written), hidden MLP layers."""
+    residual stream read, residual stream written), hidden MLP layers; with
+    fused_ln also the LayerNorm outputs those GEMMs write (LN1 of every layer,
+    LN2 of every layer)."""
     N = spec.n_elec
     C = 3 * N + 2
     D = spec.num_heads * spec.heads_dim
@@ -125,12 +138,14 @@ def psiformer_layer_bytes(spec: ModelSpec, n_atoms: int):
         dims = [D] + list(spec.mlp_hidden) + [D]
         for j, (a, b) in enumerate(zip(dims[:-1], dims[1:])):
             tot += 4.0 * N * C * (a + b + (D if j == len(dims) - 2 else 0))
+        if fused_ln:
+            tot += 2 * 4.0 * N * C * D
     return tot
 
 
-def layer_bytes(spec: ModelSpec, n_atoms: int):
+def layer_bytes(spec: ModelSpec, n_atoms: int, precision: str = "fp32"):
     return (ferminet_layer_bytes(spec, n_atoms) if spec.kind == "ferminet"
-            else psiformer_layer_bytes(spec, n_atoms))
+            else psiformer_layer_bytes(spec, n_atoms, ln_fused(spec, precision)))
 
 
 MFMA_PER_PRODUCT = {"fp32": 1, "bf16": 1, "bf16x3": 3, "bf16x6": 6, "fp16x3": 3}
