@@ -349,8 +349,7 @@ int jq_model::orbitals_and_dets(const float* elec, const void* Xlast, int Bc, bo
     go.rows = (int)(nu * RP);
     go.K = korb;
     go.ncols = K * N;
-    go.R = 1;
-    go.lrp = lap ? RP : 0;
+    go.R = lap ? RP : 1;  // Laplacian mode: unit-tiled launch
     go.act = ACT_NONE;
     ProfScope ps(lap ? PROF_GEMM_ORB_LAP : -1, s);
     if (launch_gemm(go, prec, false, s)) return jq_fail(JQ_EINVAL, "orbital GEMM launch rejected");

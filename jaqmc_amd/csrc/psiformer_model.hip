@@ -425,7 +425,7 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     go.lrp = lap ? RP : 0;
     go.act = ACT_NONE;
     go.bias = borb[c];
-    if (lap && borb[c]) {  // orbital bias on the value rows only: unit-tiled launch
+    if (lap) {  // unit-tiled launch (the bias, if any, on the value rows only)
       go.R = RP;
       go.wmaj = 1;
       go.Bc = Bc;
