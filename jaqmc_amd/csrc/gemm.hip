@@ -69,7 +69,9 @@ int launch_gemm(const GemmArgs& a, int prec, bool y_bf16, hipStream_t s) {
 
 // Benchmark hook: time `reps` launches of one GEMM configuration on synthetic
 // device buffers (HIP events on the null stream).  variant % 100 as
-// g_gemm_variant; variant >= 100 selects the plain row-tile-major order.
+// g_gemm_variant; variant % 1000 >= 100 selects the plain row-tile-major
+// order; variant / 1000 > 0 sets the tangent-unit rows of plain (R = 1)
+// launches (48 otherwise).
 extern "C" int jq_bench_gemm(int prec, int R, int rows, int K, int ncols, int act, int variant, int reps,
                              float* ms_out) {
   if (rows <= 0 || K <= 0 || ncols <= 0 || K % 32 || (R > 1 && rows % R) || reps <= 0 || prec < 0 ||
@@ -125,8 +127,8 @@ extern "C" int jq_bench_gemm(int prec, int R, int rows, int K, int ncols, int ac
       a.Res = X;
       a.ldres = K;
     }
-    if (R <= 1) a.lrp = 48;  // plain rows laid out as 48-row tangent units
-    a.linear_tiles = variant >= 100 ? 1 : 0;
+    if (R <= 1) a.lrp = variant >= 1000 ? variant / 1000 : 48;  // plain rows as lrp-row tangent units
+    a.linear_tiles = variant % 1000 >= 100 ? 1 : 0;
     const int old = g_gemm_variant;
     g_gemm_variant = variant % 100;
     hipEvent_t e0, e1;
