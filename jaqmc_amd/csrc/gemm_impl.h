@@ -572,6 +572,12 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // head-split destination: column part ((part*H + h)*N*RU)*dh + d is per lane,
   // the row part ((w*3*H*N + i)*RU + rr)*dh per row; Laplacian-mode units of the
   // tile are consecutive, so (w, i) of its first unit is computed once.
+  // plain mode: the lane's four bias values, the same for every row it stores
+  float bq[4] = {0.f, 0.f, 0.f, 0.f};
+  if (!LAP && a.bias) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bq[q] = n + q < a.ncols ? a.bias[n + q] : 0.f;
+  }
   long hcol = 0;
   int hw0 = 0, hi0 = 0;
   if (!LNORM && a.hs_H > 0) {
@@ -612,17 +618,24 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       *(float4*)y = *(const float4*)&stg[r * SW + c4];
       if (!LAP) {
         // plain mode: broadcast term, bias and activation
-        const float* gp = nullptr;
         if (a.G) {
           const long w = a.wmaj ? grow / (a.n_up + a.n_dn) : walker_of_unit((int)grow, a.Bc, a.n_up, a.n_dn);
-          gp = a.G + w * a.ldg + n;
+          const float* gp = a.G + w * a.ldg + n;
+          if (!tail && a.ldg % 4 == 0) {
+            const float4 g4 = *(const float4*)gp;
+            y[0] += g4.x;
+            y[1] += g4.y;
+            y[2] += g4.z;
+            y[3] += g4.w;
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+              if (n + q < a.ncols) y[q] += gp[q];
+          }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (n + q < a.ncols) {
-            if (gp) y[q] += gp[q];
-            if (a.bias) y[q] += a.bias[n + q];
-          }
+          y[q] += bq[q];
           if (a.act == ACT_TANH) y[q] = tanhf(y[q]);
         }
       }
