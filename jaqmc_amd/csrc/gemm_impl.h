@@ -589,8 +589,25 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       hi0 = u0 % a.hs_N;
     }
   }
+  // Residual rows (vectorised case) are requested one 16-row block ahead, so
+  // a block's stores do not wait for its own residual loads.
+  constexpr int IPB = 16 / RPI;  // rows per lane and block
+  const bool res_vec = Res != nullptr && !tail;
+  float4 rqn[IPB];
+  auto load_res = [&](int mb, float4* out) __attribute__((always_inline)) {
+#pragma unroll
+    for (int it = 0; it < IPB; ++it) {
+      const long grow = row0 + mb * 16 + r_lane + RPI * it;
+      out[it] = (res_vec && grow < a.rows) ? *(const float4*)(Res + grow * a.ldres + n) : float4{0.f, 0.f, 0.f, 0.f};
+    }
+  };
+  load_res(0, rqn);
 #pragma unroll 1
   for (int mb0 = 0; mb0 < MB; ++mb0) {
+    float4 rq[IPB];
+#pragma unroll
+    for (int it = 0; it < IPB; ++it) rq[it] = rqn[it];
+    if (mb0 + 1 < MB) load_res(mb0 + 1, rqn);
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)
       if (mb == mb0) {
@@ -662,11 +679,11 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
         if (Res) {
           const float* rp = Res + grow * a.ldres + n;
           if (!tail) {
-            const float4 rq = *(const float4*)rp;
-            y[0] = (y[0] + rq.x) * a.res_scale;
-            y[1] = (y[1] + rq.y) * a.res_scale;
-            y[2] = (y[2] + rq.z) * a.res_scale;
-            y[3] = (y[3] + rq.w) * a.res_scale;
+            const float4 r4 = rq[it];
+            y[0] = (y[0] + r4.x) * a.res_scale;
+            y[1] = (y[1] + r4.y) * a.res_scale;
+            y[2] = (y[2] + r4.z) * a.res_scale;
+            y[3] = (y[3] + r4.w) * a.res_scale;
           } else {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
