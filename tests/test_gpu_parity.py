@@ -77,11 +77,20 @@ EDGE_CASES = {
     "li_polarized": (_polarized_li, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=3)),
 }
 
+def _ring41():
+    r = S.ring42()  # without its last hydrogen: 41 electrons
+    return S.make_system("ring41", r.atoms[:11], r.charges[:11])
+
+
 PSI_CASES = {
     # 6-electron piece of the ring, narrow Psiformer (heads_dim multiple of 32)
     "psi_small": (_ring6, dict(num_layers=2, num_heads=2, heads_dim=32, mlp_hidden=[32], ndets=2)),
     # full 42-electron ring with the 4-layer, 4-head, 256-wide Psiformer
     "ring42": (None, {}),
+    # the ring less one hydrogen: 3N + 2 = 125 tangent rows in 128-row units (padding
+    # rows), one MLP layer (the fused LayerNorm of the next layer written over
+    # the MLP's own input)
+    "psi41": (_ring41, dict(num_layers=2, mlp_hidden=[], ndets=2)),
 }
 
 
@@ -330,6 +339,31 @@ def test_mixed_precision_large_configs(cuda, case, B, prec):
     rel, bnd = el_report(f"{case} B={B}", out["e_loc"].cpu().numpy(), ref["e_loc"], ref["kappa"], prec,
                          ref["cancel"])
     assert np.all(rel <= bnd), (rel, bnd)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16x3"])
+def test_psiformer_padded_units_and_fused_layernorm(cuda, prec):
+    """A 41-electron Psiformer (125 tangent rows in 128-row units, 256
+    features, one MLP layer): in FP16X3 every LayerNorm runs in the epilogue
+    of the preceding GEMM, including the one written over the MLP GEMM's own
+    input, and the padding rows stay zero; log|psi| and E_L against the FP64
+    oracle with the kappa-scaled bounds of the mode."""
+    import torch
+
+    sysm, spec, p, x, ref = _setup("psi41", 4)
+    dm = _model(sysm, spec, p, prec=prec)
+    xe = torch.from_numpy(x).to(cuda)
+    lp = dm.logpsi(xe)[0]
+    out = dm.local_energy(xe, with_grad=True)
+    torch.cuda.synchronize()
+    d_lp = np.abs(lp.cpu().numpy() - ref["logpsi"])
+    assert np.all(d_lp <= logpsi_bound(ref["logpsi"], ref["kappa"], prec, ref["cancel"])), (d_lp, ref["kappa"])
+    d_lap = np.abs(out["logpsi"].cpu().numpy() - ref["logpsi"])
+    assert np.all(d_lap <= logpsi_bound(ref["logpsi"], ref["kappa"], prec, ref["cancel"])), (d_lap, ref["kappa"])
+    rel, bnd = el_report(f"psi41 B=4", out["e_loc"].cpu().numpy(), ref["e_loc"], ref["kappa"], prec,
+                         ref["cancel"])
+    assert np.all(rel <= bnd), (rel, bnd)
+    dm.close()
 
 
 def test_empty_batch_is_a_no_op(cuda):
