@@ -560,7 +560,122 @@ int launch_mode(const AttnArgs& a, hipStream_t s) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Value-only attention (the Metropolis forward pass): s = q.k (q carries the
+// 1/sqrt(dh) scale), a = softmax_j(s), o = a.v (backbone/psiformer.py:29-60),
+// FP32 on the VALU, one workgroup per (walker, head), nothing but Q, K, V and
+// O in HBM.  The head's Q, K, V rows ([part][h][i][dh] of the head-split
+// projection, 16-byte loads) and the score matrix live in LDS (rows padded by
+// 4 floats).  Scores: each thread forms a 2 x 4 block (two query rows, four
+// key rows) from 16-byte LDS reads; softmax: one wavefront per query row
+// (lane j holds key j, DPP maximum and sum); output: each thread forms a
+// 2 x 4 block (two query rows, four features) over the keys.
+namespace {
+
+template <int DH>
+__global__ __launch_bounds__(256) void attn_fwd_kernel(const float* __restrict__ QKV, float* __restrict__ O, int ldo,
+                                                       int N, int H) {
+  constexpr int SR = DH + 4;  // Q, K, V row stride (floats)
+  constexpr int D4 = DH / 4;
+  extern __shared__ float fsm[];
+  const int NP = (N + 3) & ~3, SS = NP + 4;
+  float* Qs = fsm;
+  float* Ks = Qs + N * SR;
+  float* Vs = Ks + N * SR;
+  float* Ss = Vs + N * SR;  // [i][j], stride SS
+  const int z = blockIdx.x, w = z / H, h = z % H;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long blk = (long)N * DH;  // one (part, head) block
+  const float* src = QKV + ((long)w * 3 * H + h) * blk;
+  for (int e = tid; e < 3 * N * D4; e += 256) {
+    const int part = e / (N * D4), r = e % (N * D4), i = r / D4, q = r % D4;
+    const float4 v = *(const float4*)(src + (long)part * H * blk + (long)i * DH + 4 * q);
+    *(float4*)&Qs[part * N * SR + i * SR + 4 * q] = v;
+  }
+  __syncthreads();
+  // scores, 2 x 4 blocks; rows past N read row N - 1 and are not stored
+  const int NI = (N + 1) / 2, NJ = NP / 4;
+  for (int it = tid; it < NI * NJ; it += 256) {
+    const int i0 = 2 * (it / NJ), j0 = 4 * (it % NJ);
+    const int i1 = min(i0 + 1, N - 1);
+    int jr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) jr[q] = min(j0 + q, N - 1);
+    float acc[2][4] = {};
+#pragma unroll 4
+    for (int d = 0; d < DH; d += 4) {
+      const float4 a0 = *(const float4*)&Qs[i0 * SR + d], a1 = *(const float4*)&Qs[i1 * SR + d];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 b = *(const float4*)&Ks[jr[q] * SR + d];
+        acc[0][q] += a0.x * b.x + a0.y * b.y + a0.z * b.z + a0.w * b.w;
+        acc[1][q] += a1.x * b.x + a1.y * b.y + a1.z * b.z + a1.w * b.w;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      Ss[i0 * SS + j0 + q] = acc[0][q];
+      if (i0 + 1 < N) Ss[(i0 + 1) * SS + j0 + q] = acc[1][q];
+    }
+  }
+  __syncthreads();
+  // softmax over the keys of each query row (N <= 64: lane j holds key j)
+  for (int i = wave; i < N; i += 4) {
+    const float sj = lane < N ? Ss[i * SS + lane] : -INFINITY;
+    float m = sj;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    const float e = lane < N ? expf(sj - m) : 0.f;
+    const float sum = wave_sum(e);
+    if (lane < NP) Ss[i * SS + lane] = lane < N ? e / sum : 0.f;
+  }
+  __syncthreads();
+  // output rows, 2 x 4 blocks over the features
+  float* ob = O + (long)w * N * ldo + h * DH;
+  for (int it = tid; it < NI * D4; it += 256) {
+    const int i0 = 2 * (it / D4), d = 4 * (it % D4);
+    const int i1 = min(i0 + 1, N - 1);
+    float4 o0 = {0.f, 0.f, 0.f, 0.f}, o1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int j = 0; j < N; ++j) {
+      const float p0 = Ss[i0 * SS + j], p1 = Ss[i1 * SS + j];
+      const float4 v = *(const float4*)&Vs[j * SR + d];
+      o0.x += p0 * v.x;
+      o0.y += p0 * v.y;
+      o0.z += p0 * v.z;
+      o0.w += p0 * v.w;
+      o1.x += p1 * v.x;
+      o1.y += p1 * v.y;
+      o1.z += p1 * v.z;
+      o1.w += p1 * v.w;
+    }
+    *(float4*)(ob + (long)i0 * ldo + d) = o0;
+    if (i0 + 1 < N) *(float4*)(ob + (long)(i0 + 1) * ldo + d) = o1;
+  }
+}
+
+}  // namespace
+
 bool attn_lap_supported(int N, int dh) { return N >= 1 && N <= 48 && (dh == 32 || dh == 64); }
+
+bool attn_fwd_supported(int N, int dh) { return N >= 1 && N <= 64 && (dh == 32 || dh == 64); }
+
+int launch_attn_fwd(const AttnArgs& a, hipStream_t s) {
+  if (!attn_fwd_supported(a.N, a.dh) || a.nz <= 0 || a.Rp != 1 || a.ldo % 4) return -1;
+  const int NP = (a.N + 3) & ~3;
+  const size_t smem = ((size_t)3 * a.N * (a.dh + 4) + (size_t)a.N * (NP + 4)) * sizeof(float);  // <= 70 KB
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)attn_fwd_kernel<32>, hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024);
+    (void)hipFuncSetAttribute((const void*)attn_fwd_kernel<64>, hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024);
+    attr = true;
+  }
+  if (a.dh == 32)
+    hipLaunchKernelGGL(attn_fwd_kernel<32>, dim3(a.nz), dim3(256), smem, s, a.QKV, a.O, a.ldo, a.N, a.H);
+  else
+    hipLaunchKernelGGL(attn_fwd_kernel<64>, dim3(a.nz), dim3(256), smem, s, a.QKV, a.O, a.ldo, a.N, a.H);
+  return 0;
+}
 
 int launch_attn_lap(const AttnArgs& a, hipStream_t s) {
   if (!attn_lap_supported(a.N, a.dh) || a.nz <= 0 || a.R > a.Rp || a.R < ROW_J0 || a.ldo % 4) return -1;

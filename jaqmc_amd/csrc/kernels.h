@@ -179,3 +179,6 @@ struct AttnArgs {
 };
 bool attn_lap_supported(int N, int dh);
 int launch_attn_lap(const AttnArgs& a, hipStream_t s);
+// Value-only attention of the forward pass (Rp = 1): O[w][i][h*dh + d].
+bool attn_fwd_supported(int N, int dh);
+int launch_attn_fwd(const AttnArgs& a, hipStream_t s);

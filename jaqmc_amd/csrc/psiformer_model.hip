@@ -297,6 +297,18 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
       aa.nz = Bc * H;
       aa.h3 = prec == PREC_FP16X3 ? 1 : 0;
       rc |= launch_attn_lap(aa, s);
+    } else if (!lap && attn_fwd_supported(N, dh)) {
+      AttnArgs aa{};
+      aa.QKV = QKV;
+      aa.O = Om;
+      aa.ldo = DO;
+      aa.N = N;
+      aa.Rp = 1;
+      aa.R = 1;
+      aa.H = H;
+      aa.dh = dh;
+      aa.nz = Bc * H;
+      rc |= launch_attn_fwd(aa, s);
     } else {
       ProfScope ps(lap ? PROF_ATTN_LAP : -1, s);
       const long blk = (long)N * RP * dh;   // one (w, part, h) block of QKV
