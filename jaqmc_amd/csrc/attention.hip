@@ -41,6 +41,8 @@
 // they are combined.  The FP16 planes take the LDS the FP32 rows took; V, the
 // weights and every elementwise step stay FP32.
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace {
@@ -619,15 +621,42 @@ __global__ __launch_bounds__(256) void attn_fwd_kernel(const float* __restrict__
     }
   }
   __syncthreads();
-  // softmax over the keys of each query row (N <= 64: lane j holds key j)
-  for (int i = wave; i < N; i += 4) {
-    const float sj = lane < N ? Ss[i * SS + lane] : -INFINITY;
-    float m = sj;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    const float e = lane < N ? expf(sj - m) : 0.f;
-    const float sum = wave_sum(e);
-    if (lane < NP) Ss[i * SS + lane] = lane < N ? e / sum : 0.f;
+  // softmax over the keys of each query row (N <= 64: lane j holds key j),
+  // two rows at a time; maximum and sum are 16-lane DPP reductions combined
+  // through scalar registers
+  auto rl = [](float v, int l) __attribute__((always_inline)) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+  };
+  auto dpp = [](float v, auto ctl) __attribute__((always_inline)) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), decltype(ctl)::value, 0xF,
+                                                              0xF, false));
+  };
+  using Q1 = std::integral_constant<int, 0xB1>;
+  using Q2 = std::integral_constant<int, 0x4E>;
+  using HM = std::integral_constant<int, 0x141>;
+  using RM = std::integral_constant<int, 0x140>;
+  auto wmax = [&](float v) __attribute__((always_inline)) {
+    v = fmaxf(v, dpp(v, Q1{}));
+    v = fmaxf(v, dpp(v, Q2{}));
+    v = fmaxf(v, dpp(v, HM{}));
+    v = fmaxf(v, dpp(v, RM{}));
+    return fmaxf(fmaxf(rl(v, 0), rl(v, 16)), fmaxf(rl(v, 32), rl(v, 48)));
+  };
+  auto wsum = [&](float v) __attribute__((always_inline)) {
+    v = row16_sum(v);
+    return (rl(v, 0) + rl(v, 16)) + (rl(v, 32) + rl(v, 48));
+  };
+  for (int i0 = 2 * wave; i0 < N; i0 += 8) {
+    const int i1 = i0 + 1 < N ? i0 + 1 : i0;
+    const float s0 = lane < N ? Ss[i0 * SS + lane] : -INFINITY;
+    const float s1 = lane < N ? Ss[i1 * SS + lane] : -INFINITY;
+    const float m0 = wmax(s0), m1 = wmax(s1);
+    const float e0 = lane < N ? expf(s0 - m0) : 0.f, e1 = lane < N ? expf(s1 - m1) : 0.f;
+    const float sum0 = wsum(e0), sum1 = wsum(e1);
+    if (lane < NP) {
+      Ss[i0 * SS + lane] = lane < N ? e0 / sum0 : 0.f;
+      if (i0 + 1 < N) Ss[(i0 + 1) * SS + lane] = lane < N ? e1 / sum1 : 0.f;
+    }
   }
   __syncthreads();
   // output rows, 2 x 4 blocks over the features
