@@ -397,63 +397,84 @@ __global__ void spin_means_kernel(FermiDims d, const T* __restrict__ X, long ldx
 // sigma.  Gradient wrt x_i: sum_I -a f (x_i - R_I)/r; Laplacian: sum_I f (a^2 - 2a/r).
 // Output env[w][i][k*N + o][5] = (value, Laplacian, d/dx, d/dy, d/dz) in
 // Laplacian mode, env[w][i][k*N + o] = value in forward mode.
-// One workgroup per (walker, electron): the electron-nucleus displacements are
-// computed once into LDS, and the threads walk (k, o) with the parameters laid
-// out [I][k][o] (coalesced across the threads for each nucleus).
+// One workgroup per electron index i and block of EW walkers (one spin
+// channel, so one parameter set): the EW electrons' displacements to the
+// nuclei are computed once into LDS, and the threads walk (k, o) with the
+// parameters laid out [I][k][o] (coalesced across the threads for each
+// nucleus), each parameter pair loaded once for the EW electrons.
+constexpr int kEnvWalkers = 8;
+
 template <bool LAP>
 __global__ __launch_bounds__(256) void envelope_kernel(FermiDims d, const float* __restrict__ elec,
                                                        const float* __restrict__ atoms, const float* __restrict__ pi0,
                                                        const float* __restrict__ sg0, const float* __restrict__ pi1,
                                                        const float* __restrict__ sg1, int kind,
                                                        float* __restrict__ env) {
-  extern __shared__ float geo[];  // [M][4]: dx, dy, dz, r; then [M]: 1 / r
-  const int wi = blockIdx.x, i = wi % d.N;
-  const int KN = d.K * d.N;
+  constexpr int EW = kEnvWalkers;
+  extern __shared__ float geo[];  // [EW][M][4]: dx, dy, dz, r log2(e); then [EW][M]: 1 / r
+  const int i = blockIdx.x, w0 = blockIdx.y * EW;
+  const int ne = min(EW, d.Bc - w0);
+  const int KN = d.K * d.N, M = d.M;
   constexpr int ES = LAP ? 5 : 1;  // env_stride
-  float* out = env + (long)wi * KN * ES;
   if (kind == 2) {  // null envelope
-    for (int ko = threadIdx.x; ko < KN; ko += blockDim.x) {
-      out[ko * ES] = 1.f;
-      if constexpr (LAP) out[ko * 5 + 1] = out[ko * 5 + 2] = out[ko * 5 + 3] = out[ko * 5 + 4] = 0.f;
+    for (int e = 0; e < ne; ++e) {
+      float* out = env + ((long)(w0 + e) * d.N + i) * KN * ES;
+      for (int ko = threadIdx.x; ko < KN; ko += blockDim.x) {
+        out[ko * ES] = 1.f;
+        if constexpr (LAP) out[ko * 5 + 1] = out[ko * 5 + 2] = out[ko * 5 + 3] = out[ko * 5 + 4] = 0.f;
+      }
     }
     return;
   }
-  const float* xi = elec + (long)wi * 3;
-  for (int I = threadIdx.x; I < d.M; I += blockDim.x) {
+  float* const irs = geo + 4 * EW * M;
+  for (int t = threadIdx.x; t < EW * M; t += blockDim.x) {
+    const int e = t / M, I = t % M;
+    const float* xi = elec + ((long)(w0 + (e < ne ? e : 0)) * d.N + i) * 3;  // past the last walker: a copy
     const float dx = xi[0] - atoms[3 * I], dy = xi[1] - atoms[3 * I + 1], dz = xi[2] - atoms[3 * I + 2];
-    geo[4 * I] = dx;
-    geo[4 * I + 1] = dy;
-    geo[4 * I + 2] = dz;
+    geo[4 * t] = dx;
+    geo[4 * t + 1] = dy;
+    geo[4 * t + 2] = dz;
     const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-    geo[4 * I + 3] = r;
-    geo[4 * d.M + I] = 1.f / r;
+    geo[4 * t + 3] = r * 1.4426950408889634f;  // r log2(e): exp(-a r) = 2^(-a r log2 e)
+    irs[t] = 1.f / r;
   }
   __syncthreads();
   const bool up = d.nch == 1 || i < d.n_up;
   const float* pi = up ? pi0 : pi1;
   const float* sg = up ? sg0 : sg1;
   for (int ko = threadIdx.x; ko < KN; ko += blockDim.x) {
-    float v = 0.f, l = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-    for (int I = 0; I < d.M; ++I) {
-      const float4 g = *(const float4*)&geo[4 * I];
-      const float ir = geo[4 * d.M + I];  // one division per atom and workgroup
+    float v[EW], l[EW], gx[EW], gy[EW], gz[EW];
+#pragma unroll
+    for (int e = 0; e < EW; ++e) v[e] = l[e] = gx[e] = gy[e] = gz[e] = 0.f;
+    for (int I = 0; I < M; ++I) {
       const float a = kind == 1 ? fabsf(sg[(long)I * KN + ko]) : sg[(long)I * KN + ko];
-      const float f = pi[(long)I * KN + ko] * expf(-a * g.w);
-      v += f;
-      if constexpr (LAP) {
-        l += f * a * (a - 2.f * ir);
-        const float gs = -a * f * ir;
-        gx += gs * g.x;
-        gy += gs * g.y;
-        gz += gs * g.z;
+      const float p = pi[(long)I * KN + ko];
+#pragma unroll
+      for (int e = 0; e < EW; ++e) {
+        const float4 g = *(const float4*)&geo[4 * (e * M + I)];
+        const float f = p * __builtin_amdgcn_exp2f(-a * g.w);
+        v[e] += f;
+        if constexpr (LAP) {
+          const float ir = irs[e * M + I];  // one division per atom and electron
+          l[e] += f * a * (a - 2.f * ir);
+          const float gs = -a * f * ir;
+          gx[e] += gs * g.x;
+          gy[e] += gs * g.y;
+          gz[e] += gs * g.z;
+        }
       }
     }
-    out[ko * ES] = v;
-    if constexpr (LAP) {
-      out[ko * 5 + 1] = l;
-      out[ko * 5 + 2] = gx;
-      out[ko * 5 + 3] = gy;
-      out[ko * 5 + 4] = gz;
+#pragma unroll
+    for (int e = 0; e < EW; ++e) {
+      if (e >= ne) break;
+      float* out = env + ((long)(w0 + e) * d.N + i) * KN * ES;
+      out[ko * ES] = v[e];
+      if constexpr (LAP) {
+        out[ko * 5 + 1] = l[e];
+        out[ko * 5 + 2] = gx[e];
+        out[ko * 5 + 3] = gy[e];
+        out[ko * 5 + 4] = gz[e];
+      }
     }
   }
 }
@@ -598,13 +619,14 @@ int launch_spin_means(const FermiDims& d, const void* X, long ldx, int f1, void*
 int launch_envelope(const FermiDims& d, const float* elec, const float* atoms, const float* const* pi,
                     const float* const* sigma, int env_kind, float* env, hipStream_t s) {
   if ((long)d.Bc * d.N <= 0) return 0;
-  const size_t smem = (size_t)5 * d.M * sizeof(float);
+  const size_t smem = (size_t)5 * kEnvWalkers * d.M * sizeof(float);
   if (smem > 64 * 1024) return -1;
+  const dim3 grid((unsigned)d.N, (unsigned)((d.Bc + kEnvWalkers - 1) / kEnvWalkers));
   if (env_stride(d) == 5)
-    hipLaunchKernelGGL(envelope_kernel<true>, dim3((unsigned)(d.Bc * d.N)), 256, smem, s, d, elec, atoms, pi[0],
-                       sigma[0], d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
+    hipLaunchKernelGGL(envelope_kernel<true>, grid, 256, smem, s, d, elec, atoms, pi[0], sigma[0],
+                       d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
   else
-    hipLaunchKernelGGL(envelope_kernel<false>, dim3((unsigned)(d.Bc * d.N)), 256, smem, s, d, elec, atoms, pi[0],
-                       sigma[0], d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
+    hipLaunchKernelGGL(envelope_kernel<false>, grid, 256, smem, s, d, elec, atoms, pi[0], sigma[0],
+                       d.nch > 1 ? pi[1] : pi[0], d.nch > 1 ? sigma[1] : sigma[0], env_kind, env);
   return 0;
 }
