@@ -205,8 +205,15 @@ def chunk_for_budget(spec: ModelSpec, atoms, charges, params, precision, walkers
         m = DeviceModel(spec, atoms, charges, params, precision=precision, chunk_walkers=c)
         ws.append(m.workspace_bytes)
         m.close()
-    per = max(1, ws[1] - ws[0])
-    fixed = ws[0] - per
+    return equal_chunks(ws[0], ws[1], walkers, budget_bytes)
+
+
+def equal_chunks(ws1: int, ws2: int, walkers: int, budget_bytes) -> int:
+    """Walkers per chunk from the workspace bytes at chunks of one and two
+    walkers (affine in the chunk): the fewest equal chunks whose workspace fits
+    budget_bytes (at least one walker per chunk)."""
+    per = max(1, int(ws2) - int(ws1))
+    fixed = int(ws1) - per
     cmax = max(1, (int(budget_bytes) - fixed) // per)
     n = -(-int(walkers) // cmax)
     return -(-int(walkers) // n)

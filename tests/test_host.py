@@ -147,3 +147,18 @@ def test_bench_gemm_broadcast_rows_stay_in_bounds():
         Bc = units // 14 if units // 14 > 0 else 1
         top = max(walker_of_unit(u, Bc, 7, 7) for u in range(units))
         assert top < units // 7 + 2
+
+
+def test_equal_chunks_from_workspace_probe():
+    """Chunk sizing for a workspace budget: affine workspace (fixed + per
+    walker), fewest equal chunks that fit, at least one walker per chunk."""
+    from jaqmc_amd.device import equal_chunks
+
+    fixed, per = 1000, 100
+    ws1, ws2 = fixed + per, fixed + 2 * per
+    assert equal_chunks(ws1, ws2, 2048, fixed + 2048 * per) == 2048  # everything fits
+    assert equal_chunks(ws1, ws2, 2048, fixed + 2047 * per) == 1024  # two equal chunks
+    assert equal_chunks(ws1, ws2, 2048, fixed + 700 * per) == 683    # three chunks: 683, 683, 682
+    assert equal_chunks(ws1, ws2, 10, 0) == 1                         # budget below one walker
+    c = equal_chunks(ws1, ws2, 1000, fixed + 333 * per)
+    assert fixed + c * per <= fixed + 333 * per and -(-1000 // c) == 4
