@@ -188,6 +188,17 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       }
     }
   };
+  // the 16-bit fragments of column blocks [nb0, nb0 + nbn) only
+  auto bload_part = [&](auto set, int k0, int nb0, int nbn) __attribute__((always_inline)) {
+    constexpr int S = decltype(set)::value;
+#pragma unroll
+    for (int nb = nb0; nb < nb0 + nbn; ++nb) {
+      const int gc = col0 + wcol + nb * 16 + lr;
+      const long gcl = gc < a.ncols ? gc : 0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) bnb[S][s][nb] = *(const uint4*)(Wb + s * a.wplane + gcl * a.ldw + k0 + 8 * lg);
+    }
+  };
   auto lstore = [&](int buf) __attribute__((always_inline)) {
     TL* const Ab = Abase + buf * NSL * PLANE;
 #pragma clang loop unroll(full)
@@ -336,8 +347,22 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     }
   }
 
-  // products of one chunk from operand tile `buf` with B fragment set `set`
-  auto mfma_chunk = [&](auto set, int buf) __attribute__((always_inline)) {
+  // EARLY_B: the next chunk's B fragments are requested into the other set
+  // before this chunk's products, so no fragment is ever copied and no wait
+  // covers loads issued after them.  Otherwise (three planes, or tiles of
+  // four column blocks per wave in FP32, where two sets do not fit the
+  // register budget) one set is used and the next chunk's fragments are
+  // requested after their last product.  SPLITB (FP16X3, four column blocks
+  // per wave, two operand tiles): a chunk's products run as two column
+  // halves, each over every row block (the A fragments are read from LDS
+  // once per half), so the next chunk's fragments of the first half are
+  // requested half a chunk earlier.
+  constexpr bool EARLY_B = F32 ? NBW <= 2 : NS * NBW <= 4;
+  constexpr bool SPLITB = F16 && DB && !EARLY_B && NBW == 4;
+  // products of one chunk from operand tile `buf` with B fragment set `set`;
+  // SPLITB requests the next chunk's (at knext) first-half fragments after
+  // the first half's products
+  auto mfma_chunk = [&](auto set, int buf, int knext) __attribute__((always_inline)) {
     constexpr int S = decltype(set)::value;
     const TL* const Ab = Abase + buf * NSL * PLANE;
     if constexpr (F32) {
@@ -353,48 +378,51 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
                                                                bn32[S][nb][kk >> 2][kk & 3], acc[mb][nb], 0, 0, 0);
       }
     } else {
+      constexpr int NH = SPLITB ? 2 : 1;  // column halves
+      constexpr int NBH = NBW / NH;
 #pragma unroll
-      for (int mb = 0; mb < MB; ++mb) {
-        bf16x8 at[NS];
+      for (int h = 0; h < NH; ++h) {
+        if (h > 0) {
+          // the second half reads its A fragments again (kept from the first
+          // half they would take 64 registers)
+          __asm__ volatile("" ::: "memory");
+          bload_part(set, knext, 0, NBH);
+        }
 #pragma unroll
-        for (int s = 0; s < NS; ++s) at[s] = *(const bf16x8*)&Ab[s * PLANE + (mb * 16 + lr) * LS + lg * 8];
+        for (int mb = 0; mb < MB; ++mb) {
+          bf16x8 at[NS];
 #pragma unroll
-        for (int nb = 0; nb < NBW; ++nb) {
-          if constexpr (F16) {
+          for (int s = 0; s < NS; ++s) at[s] = *(const bf16x8*)&Ab[s * PLANE + (mb * 16 + lr) * LS + lg * 8];
+#pragma unroll
+          for (int nb = h * NBH; nb < (h + 1) * NBH; ++nb) {
+            if constexpr (F16) {
+              // small terms first
+              const f16x8 a0 = __builtin_bit_cast(f16x8, at[0]), a1 = __builtin_bit_cast(f16x8, at[1]);
+              const f16x8 b0 = __builtin_bit_cast(f16x8, bnb[S][0][nb]), b1 = __builtin_bit_cast(f16x8, bnb[S][1][nb]);
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b0, acc[mb][nb], 0, 0, 0);
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b1, acc[mb][nb], 0, 0, 0);
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b0, acc[mb][nb], 0, 0, 0);
+              continue;
+            }
+            bf16x8 bt[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) bt[s] = __builtin_bit_cast(bf16x8, bnb[S][s][nb]);
             // small terms first
-            const f16x8 a0 = __builtin_bit_cast(f16x8, at[0]), a1 = __builtin_bit_cast(f16x8, at[1]);
-            const f16x8 b0 = __builtin_bit_cast(f16x8, bnb[S][0][nb]), b1 = __builtin_bit_cast(f16x8, bnb[S][1][nb]);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a1, b0, acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b1, acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a0, b0, acc[mb][nb], 0, 0, 0);
-            continue;
+            if constexpr (NS == 3) {
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[1], acc[mb][nb], 0, 0, 0);
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[2], acc[mb][nb], 0, 0, 0);
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[2], bt[0], acc[mb][nb], 0, 0, 0);
+            }
+            if constexpr (NS >= 2) {
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[1], acc[mb][nb], 0, 0, 0);
+              acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[0], acc[mb][nb], 0, 0, 0);
+            }
+            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0], acc[mb][nb], 0, 0, 0);
           }
-          bf16x8 bt[NS];
-#pragma unroll
-          for (int s = 0; s < NS; ++s) bt[s] = __builtin_bit_cast(bf16x8, bnb[S][s][nb]);
-          // small terms first
-          if constexpr (NS == 3) {
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[1], acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[2], acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[2], bt[0], acc[mb][nb], 0, 0, 0);
-          }
-          if constexpr (NS >= 2) {
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[1], acc[mb][nb], 0, 0, 0);
-            acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[1], bt[0], acc[mb][nb], 0, 0, 0);
-          }
-          acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(at[0], bt[0], acc[mb][nb], 0, 0, 0);
         }
       }
     }
   };
-  // EARLY_B: the next chunk's B fragments are requested into the other set
-  // before this chunk's products, so no fragment is ever copied and no wait
-  // covers loads issued after them.  Otherwise (three planes, or tiles of
-  // four column blocks per wave in FP32, where two sets do not fit the
-  // register budget) one set is used and the next chunk's fragments are
-  // requested after the last product (L2 hits that arrive during the staging
-  // step).
-  constexpr bool EARLY_B = F32 ? NBW <= 2 : NS * NBW <= 4;
 
   if constexpr (!FIRST_EARLY) {
     gload(kof(0));
@@ -434,8 +462,12 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     auto step = [&](auto set, int it) __attribute__((always_inline)) {
       constexpr int S = decltype(set)::value;
       if constexpr (EARLY_B) bload(std::integral_constant<int, 1 - S>{}, kc(it + 1));
-      mfma_chunk(set, it & 1);
-      if constexpr (!EARLY_B) bload(set, kc(it + 1));
+      mfma_chunk(set, it & 1, kc(it + 1));
+      if constexpr (SPLITB) {
+        bload_part(set, kc(it + 1), NBW / 2, NBW / 2);
+      } else if constexpr (!EARLY_B) {
+        bload(set, kc(it + 1));
+      }
       lstore((it + 1) & 1);
       gload(kc(it + 2));
       __syncthreads();
@@ -455,7 +487,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       lstore(0);
       __syncthreads();
       if (it + 1 < nk) gload(kof(it + 1));
-      mfma_chunk(S0{}, 0);
+      mfma_chunk(S0{}, 0, 0);
       if (it + 1 < nk) bload(S0{}, kof(it + 1));
       __syncthreads();
     }
