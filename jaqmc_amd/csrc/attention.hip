@@ -66,9 +66,22 @@ struct AttnLayout {
   static constexpr int FLOATS = 6 * QKF + 3 * DH * SJ + 3 * NP * SJ + NT * NP + (H3 ? 6 * NP : 0);
 };
 
-// Maximum over the `n` consecutive lanes of an aligned lane group (n = 8, 16).
-__device__ inline float group_max(float v, int n) {
-  for (int o = 1; o < n; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+template <int CTL>
+__device__ inline float dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTL, 0xF, 0xF, false));
+}
+
+// Maximum over the N consecutive lanes of an aligned lane group (N = 8, 16),
+// within the 16-lane DPP row: quad_perm [1,0,3,2] and [2,3,0,1] give each
+// lane its quad's maximum, row_half_mirror (lane i reads lane 7 - i of its
+// half-row) joins the two quads of a half-row, row_mirror the two halves.
+template <int N>
+__device__ inline float group_max(float v) {
+  static_assert(N == 8 || N == 16, "lane groups of 8 or 16");
+  v = fmaxf(v, dpp_f<0xB1>(v));
+  v = fmaxf(v, dpp_f<0x4E>(v));
+  v = fmaxf(v, dpp_f<0x141>(v));
+  if constexpr (N == 16) v = fmaxf(v, dpp_f<0x140>(v));
   return v;
 }
 
@@ -110,8 +123,9 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
   float* RJ = Q0;             // after the tangent loop: rj [i][j]
   // H3: inverse row scales, one NP array per Q / K array, in the arrays' order
   // (Q_0, K_0, Q buffer 0, K buffer 0, Q buffer 1, K buffer 1)
+  // (slot s: the Q / K array at Q0 + s * QKF; Q_0 0, K_0 1, buffer b 2 + 2b, 3 + 2b)
   float* IS = Mp + NT * NP;
-  auto isc = [&](const float* arr) __attribute__((always_inline)) { return IS + (int)((arr - Q0) / QKF) * NP; };
+  auto isc = [&](int slot) __attribute__((always_inline)) { return IS + slot * NP; };
 
   const int N = a.N, RP = a.Rp, R = a.R;
   const int z = blockIdx.x, w = z / a.H, h = z % a.H;
@@ -151,8 +165,9 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
                                         : float4{0.f, 0.f, 0.f, 0.f};
     }
   };
-  // dq: a Q array, immediately followed by its K array; dv: a transposed V array
-  auto commit = [&](float* dq, float* dv) __attribute__((always_inline)) {
+  // qs: the slot of a Q array (its K array follows); dv: a transposed V array
+  auto commit = [&](int qs, float* dv) __attribute__((always_inline)) {
+    float* const dq = Q0 + qs * QKF;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       int m, i, q;
@@ -163,7 +178,7 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
         // FP16 high and low parts (every lane takes part in the shuffles)
         const float4 x = pf[p];
         float mx = (m < 2 && i < N) ? fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))) : 0.f;
-        mx = group_max(mx, F4);
+        mx = group_max<F4>(mx);
         if (m < 2 && i < N) {
           float sc = 1.f, si = 1.f;
           if (mx > 0.f && mx <= 3.0e38f) {
@@ -184,7 +199,7 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
           f16* base = (f16*)(dq + m * QKF);  // plane 0 rows, then plane 1 rows
           *(uint2*)(base + i * SQH + 4 * q) = *(const uint2*)hi;
           *(uint2*)(base + NP * SQH + i * SQH + 4 * q) = *(const uint2*)lo;
-          if (q == 0) isc(dq + m * QKF)[i] = si;
+          if (q == 0) isc(qs + m)[i] = si;
         }
         if (m == 2 && i < N) {
           dv[(4 * q + 0) * SJ + i] = x.x;
@@ -256,7 +271,7 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
   // ---- value rows and the attention weights ----
   fetch(0);
   __syncthreads();  // LDS zeroed
-  commit(Q0, V0);
+  commit(0, V0);
   fetch(1);
   __syncthreads();
   {
@@ -271,8 +286,8 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
           const int tile = wave + ATT_WAVES * t;
           if (tile < JT) mf3(acc[t], rdh(Q0, 16 * (tile / NT) + lr, ks), rdh(K0, 16 * (tile % NT) + lr, ks));
         }
-      const float* iq = isc(Q0);
-      const float* ik = isc(K0);
+      const float* iq = isc(0);
+      const float* ik = isc(1);
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
         const int tile = wave + ATT_WAVES * t;
@@ -298,7 +313,7 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
       if (tile < JT) put_tile(As, acc[t], tile / NT, tile % NT);
     }
   }
-  commit(Qb + 2 * QKF, Vb + DH * SJ);  // row 1 into buffer 1
+  commit(4, Vb + DH * SJ);  // row 1 into buffer 1
   fetch(2);
   __syncthreads();
 #pragma unroll
@@ -372,7 +387,7 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
             if (cross) mf3(a3[t], qc, kc);
           }
         }
-      const float *iqc = isc(Qc), *ikc = isc(Kc), *iq0 = isc(Q0), *ik0 = isc(K0);
+      const float *iqc = isc(2 + 2 * b), *ikc = isc(3 + 2 * b), *iq0 = isc(0), *ik0 = isc(1);
 #pragma unroll
       for (int t = 0; t < JTW; ++t) {
         const int tile = wave + ATT_WAVES * t;
@@ -450,7 +465,7 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
       }
     }
     if (c + 1 < R) {
-      commit(Qb + 2 * QKF * (b ^ 1), Vb + DH * SJ * (b ^ 1));
+      commit(2 + 2 * (b ^ 1), Vb + DH * SJ * (b ^ 1));
       fetch(c + 2);
     }
     __syncthreads();
@@ -493,7 +508,7 @@ __global__ __launch_bounds__(att_threads(H3)) void attn_lap_kernel(AttnArgs a) {
       put_tile(RJ, rjt[t], tile / NT, tile % NT);
     }
   }
-  commit(Qb, Vb);  // row 1 into buffer 0 (its readers are done)
+  commit(2, Vb);  // row 1 into buffer 0 (its readers are done)
   __syncthreads();
   // lap s = J_1 + 2 S3; lap a = a (rj - nu + lap s - lam)
 #pragma unroll

@@ -359,6 +359,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // requested half a chunk earlier.
   constexpr bool EARLY_B = F32 ? NBW <= 2 : NS * NBW <= 4;
   constexpr bool SPLITB = F16 && DB && !EARLY_B && NBW == 4;
+  constexpr int NHS = SPLITB ? 2 : 1;  // column parts of a chunk's products
   // products of one chunk from operand tile `buf` with B fragment set `set`;
   // SPLITB requests the next chunk's (at knext) first-half fragments after
   // the first half's products
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
                                                                bn32[S][nb][kk >> 2][kk & 3], acc[mb][nb], 0, 0, 0);
       }
     } else {
-      constexpr int NH = SPLITB ? 2 : 1;  // column halves
+      constexpr int NH = NHS;
       constexpr int NBH = NBW / NH;
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
           // the second half reads its A fragments again (kept from the first
           // half they would take 64 registers)
           __asm__ volatile("" ::: "memory");
-          bload_part(set, knext, 0, NBH);
+          bload_part(set, knext, (h - 1) * NBH, NBH);
         }
 #pragma unroll
         for (int mb = 0; mb < MB; ++mb) {
@@ -464,7 +465,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
       if constexpr (EARLY_B) bload(std::integral_constant<int, 1 - S>{}, kc(it + 1));
       mfma_chunk(set, it & 1, kc(it + 1));
       if constexpr (SPLITB) {
-        bload_part(set, kc(it + 1), NBW / 2, NBW / 2);
+        bload_part(set, kc(it + 1), NBW - NBW / NHS, NBW / NHS);
       } else if constexpr (!EARLY_B) {
         bload(set, kc(it + 1));
       }
