@@ -129,7 +129,15 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     rt = blockIdx.x / ncb;
     cb = blockIdx.x % ncb;
   } else {
-    xcd_tile(blockIdx.x, (a.ncols + BN - 1) / BN, &rt, &cb);
+    // XCD x = b % 8 takes the contiguous row tiles [x*tpx, (x+1)*tpx) (the
+    // grid is xcd_grid's), its column blocks of one row tile back to back:
+    // the column blocks share the tile's X panel in that XCD's L2, and the
+    // units of one walker (with their broadcast term G) stay on one XCD.
+    const int ncb = (a.ncols + BN - 1) / BN;
+    const int tpx = (int)gridDim.x / (kXcds * ncb);
+    const int x = (int)blockIdx.x % kXcds, slot = (int)blockIdx.x / kXcds;
+    cb = slot % ncb;
+    rt = x * tpx + slot / ncb;
   }
   if ((long)rt * BM >= a.rows) return;
   const long row0 = (long)rt * BM;
