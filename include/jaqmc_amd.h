@@ -58,8 +58,9 @@ extern "C" {
                                           envelope for both channels (orbital.py:73-76)            */
 #define JQ_OPT_USE_LAST_LAYER (1 << 5) /* FermiNet use_last_layer=True: rejected (JQ_EINVAL)      */
 /* Library option (no reference counterpart): allocate the parameter-gradient
- * workspace at create, for jq_logpsi_vjp (FermiNet, hidden widths multiples
- * of 32). */
+ * workspace at create, for jq_logpsi_vjp (FermiNet with hidden widths
+ * multiples of 32; Psiformer with layer_norm_mode pre or null, N <= 64 and
+ * heads_dim 32 or 64). */
 #define JQ_OPT_WITH_GRADIENTS (1 << 16)
 
 /* Arithmetic of the layer / tangent GEMMs.  Determinants (LU, inverse,
@@ -157,9 +158,9 @@ int jq_orbitals(jq_model* m, const float* elec, int B, float* orbitals, void* st
  * (estimator/loss_grad.py:70-103) reduced with the walker weights of
  * finalize_stats (:115-128; jq_loss_clip's weights give the VMC energy
  * gradient).  cot (B,) float64 on the device; a walker with cot 0 contributes
- * exactly 0 (its determinants are not inverted).  FermiNet models created with
- * JQ_OPT_WITH_GRADIENTS; FP32 arithmetic, FP64 determinant adjoint and FP64
- * accumulation over walkers. */
+ * exactly 0 (its determinants are not inverted).  FermiNet and Psiformer
+ * models created with JQ_OPT_WITH_GRADIENTS; FP32 arithmetic, FP64
+ * determinant adjoint and FP64 accumulation over walkers. */
 int jq_logpsi_vjp(jq_model* m, const float* elec, int B, const double* cot, float* grad, void* stream);
 /* Floats in the model's parameter pack (the length of jq_logpsi_vjp's grad). */
 size_t jq_model_param_count(const jq_model* m);

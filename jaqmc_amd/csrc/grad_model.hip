@@ -1,5 +1,5 @@
-// FermiNet parameter gradient: forward pass with a per-layer tape, then the
-// reverse sweep (kernels in grad.hip, dense products on the GEMM family in
+// Parameter gradients (FermiNet, Psiformer): forward pass with a per-layer
+// tape, then the reverse sweep (kernels in grad.hip, dense products on the GEMM family in
 // FP32).  See grad.hip for the derivative rules and their reference sources.
 
 #include <algorithm>
@@ -161,6 +161,12 @@ static int gemm_nt(const float* Xp, long ldx, const float* W, int K_, int ncols,
 
 int jq_model::vjp(const float* elec, int B, const double* cot, float* grad, hipStream_t s) {
   launch_fill_f64(gacc, n_params, 0.0, s);
+  if (kind == JQ_KIND_PSIFORMER) {
+    const int rc = vjp_psiformer(elec, B, cot, s);
+    if (rc) return rc;
+    launch_f64_to_f32(gacc, n_params, grad, s);
+    return 0;
+  }
   const long KN = (long)K * N;
   for (int b0 = 0; b0 < B; b0 += gchunk) {
     const int Bc = std::min(gchunk, B - b0);
@@ -232,6 +238,311 @@ int jq_model::vjp(const float* elec, int B, const double* cot, float* grad, hipS
     }
   }
   launch_f64_to_f32(gacc, n_params, grad, s);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Psiformer (pre-LN or no LayerNorm), value path:
+//   H_0 = F0 Win + bin;  per layer  XIN = LN1(H) (or H),  q|k|v = XIN Wqkv + b,
+//   O = MHA(q, k, v),  H1 = H + O Wo + bo,  M0 = LN2(H1) (or H1),
+//   m = tanh(... tanh(M0 W_0 + b_0) ... W_last + b_last),  H' = H1 + m;
+//   Phi = H_L W_orb (+ b_orb) per spin channel, then envelope and determinants.
+// The reverse sweep runs the same sequence backwards; each dense layer's
+// weight gradient is X^T dY over the chunk's rows (wgrad), its input adjoint
+// dY W^T on the FP32 GEMM with the weights in their (in, out) orientation.
+// ---------------------------------------------------------------------------
+int launch_ln_bwd(long rows, int D, const float* x, long ldx, const float* s, const float* dy, long lddy, float eps,
+                  const float* base, long ldb, float* out, long ldo, float* P, long ldp, hipStream_t st);
+size_t attn_bwd_smem(int N, int dh);
+int launch_attn_bwd(const float* QKV, const float* dO, long lddo, float* dQKV, long ldq, int Bc, int N, int H, int dh,
+                    hipStream_t st);
+int launch_jastrow_grad(const float* elec, int Bc, int N, int n_up, float ap, float aa, const double* cot, int b0,
+                        float* out, hipStream_t st);
+int launch_mask_rows(long units, int N, int i0, int i1, int ncols, const float* src, long lds, float* dst, long ldd,
+                     hipStream_t st);
+int launch_add_rows(long rows, int n, const float* a, long lda, const float* b, long ldb, float* out, long ldo,
+                    hipStream_t st);
+int launch_scale_f64(double* p, long n, double v, hipStream_t st);
+
+int jq_model::alloc_grad_workspace_psi(size_t budget) {
+  const long KN = (long)K * N;
+  const bool pre = ln_mode == JQ_LN_PRE;
+  long tape = kin + (long)(L + 1) * D, wmax = D;
+  for (const PsiLayer& ly : pl) {
+    tape += (pre ? 2L : 0L) * D + 3L * D + 2L * D;
+    for (size_t j = 1; j < ly.mdims.size(); ++j) {
+      tape += round_up(ly.mdims[j], 32);
+      wmax = std::max(wmax, (long)round_up(ly.mdims[j], 32));
+    }
+  }
+  const long back = 2L * D + 2L * wmax + wmax + D + 3L * D + D + D + 2L * ldko + KN;
+  const size_t per_walker = 4 * (size_t)N * (tape + back) + 64;
+  gchunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)chunk_fwd, budget / per_walker));
+  const size_t U = (size_t)gchunk * N;
+  auto fa = [&](size_t floats) { return (float*)alloc(std::max<size_t>(floats, 1) * 4); };
+  F0t = fa(U * kin);
+  Ht.assign(L + 1, nullptr);
+  XINt.assign(L, nullptr);
+  QKVt.assign(L, nullptr);
+  Ot.assign(L, nullptr);
+  H1t.assign(L, nullptr);
+  M0t.assign(L, nullptr);
+  MHt.assign(L, {});
+  bool ok = F0t != nullptr;
+  for (int l = 0; l <= L; ++l) ok = ok && (Ht[l] = fa(U * D)) != nullptr;
+  for (int l = 0; l < L && ok; ++l) {
+    XINt[l] = pre ? fa(U * D) : Ht[l];
+    QKVt[l] = fa(U * 3 * D);
+    Ot[l] = fa(U * D);
+    H1t[l] = fa(U * D);
+    M0t[l] = pre ? fa(U * D) : H1t[l];
+    ok = XINt[l] && QKVt[l] && Ot[l] && H1t[l] && M0t[l];
+    for (size_t j = 1; j < pl[l].mdims.size() && ok; ++j) {
+      MHt[l].push_back(fa(U * round_up(pl[l].mdims[j], 32)));
+      ok = MHt[l].back() != nullptr;
+    }
+  }
+  dH[0] = fa(U * D);
+  dH1 = fa(U * D);
+  dM[0] = fa(U * wmax);
+  dM[1] = fa(U * wmax);
+  dZ = fa(U * wmax);
+  dO = fa(U * D);
+  dQKV = fa(U * 3 * D);
+  dXin = fa(U * D);
+  Pln = fa(U * D);
+  dPhi = fa(U * ldko);
+  dPhiM = fa(U * ldko);
+  dEnv = fa(U * KN);
+  jtmp = fa(2 * (size_t)gchunk);
+  pw_cap = 16l << 20;
+  Pw = fa(pw_cap);
+  gacc = (double*)alloc((size_t)n_params * 8);
+  if (!ok || !dH[0] || !dH1 || !dM[0] || !dM[1] || !dZ || !dO || !dQKV || !dXin || !Pln || !dPhi || !dPhiM ||
+      !dEnv || !jtmp || !Pw || !gacc)
+    return jq_fail(JQ_ENOMEM, "gradient workspace allocation failed");
+  // padding columns of dPhi (KN .. ldko) are read by the orbital adjoint GEMM
+  hipStream_t st = jq_setup_stream();
+  launch_fill_f32(dPhi, (long)U * ldko, 0.f, st);
+  if (hipStreamSynchronize(st) != hipSuccess) return jq_fail(JQ_EHIP, "gradient workspace initialisation failed");
+  return 0;
+}
+
+// Value-mode Psiformer forward with the tape (the forward_psiformer sequence
+// with every intermediate kept; FP32 attention on the VALU).
+int jq_model::forward_psiformer_tape(const float* elec, int Bc, hipStream_t s) {
+  const FermiDims fd = dims(Bc, false);
+  const long units = (long)Bc * N;
+  const bool pre = ln_mode == JQ_LN_PRE;
+  auto gemm = [&](const float* X, long ldx, void* Wt, void* Wlo, long wplane, int K_, float* Y, long ldy, int ncols,
+                  const float* bias, int act, const float* Res, int zpad) {
+    GemmArgs g{};
+    g.X = X;
+    g.ldx = ldx;
+    g.Wt = Wt;
+    g.Wt_lo = Wlo;
+    g.wplane = wplane;
+    g.ldw = K_;
+    g.Y = Y;
+    g.ldy = ldy;
+    g.rows = (int)units;
+    g.K = K_;
+    g.ncols = ncols;
+    g.bias = bias;
+    g.R = 1;
+    g.act = act;
+    g.Res = Res;
+    g.ldres = ldy;
+    g.res_scale = 1.f;
+    g.Bc = Bc;
+    g.n_up = n_up;
+    g.n_dn = n_dn;
+    g.wmaj = 1;
+    g.zpad = zpad;
+    return launch_gemm(g, prec, false, s);
+  };
+  int rc = launch_psi_features(fd, elec, atoms, F0t, kin, (options & JQ_OPT_NO_RESCALE) ? 0 : 1, s);
+  rc |= gemm(F0t, kin, Win, Win_lo, (long)D * kin, kin, Ht[0], D, D, bin, ACT_NONE, nullptr, 0);
+  for (int l = 0; l < L && !rc; ++l) {
+    const PsiLayer& ly = pl[l];
+    if (pre) rc |= launch_layernorm_lap((int)units, 1, 1, D, Ht[l], XINt[l], ly.ln1_s, ly.ln1_b, 1e-5f, s);
+    GemmArgs g{};
+    g.X = XINt[l];
+    g.ldx = D;
+    g.Wt = ly.Wqkv;
+    g.Wt_lo = ly.Wqkv_lo;
+    g.wplane = 3L * D * D;
+    g.ldw = D;
+    g.Y = QKVt[l];
+    g.ldy = 3 * D;
+    g.rows = (int)units;
+    g.K = D;
+    g.ncols = 3 * D;
+    g.bias = ly.bqkv;
+    g.R = 1;
+    g.act = ACT_NONE;
+    g.res_scale = 1.f;
+    g.wmaj = 1;
+    g.hs_H = H;
+    g.hs_dh = dh;
+    g.hs_D = D;
+    g.hs_N = N;
+    rc |= launch_gemm(g, prec, false, s);
+    AttnArgs aa{};
+    aa.QKV = QKVt[l];
+    aa.O = Ot[l];
+    aa.ldo = D;
+    aa.N = N;
+    aa.Rp = 1;
+    aa.R = 1;
+    aa.H = H;
+    aa.dh = dh;
+    aa.nz = Bc * H;
+    rc |= launch_attn_fwd(aa, s);
+    rc |= gemm(Ot[l], D, ly.Wo, ly.Wo_lo, (long)D * D, D, H1t[l], D, D, ly.bo, ACT_NONE, Ht[l], 0);
+    if (pre) rc |= launch_layernorm_lap((int)units, 1, 1, D, H1t[l], M0t[l], ly.ln2_s, ly.ln2_b, 1e-5f, s);
+    const float* cur = M0t[l];
+    int curw = D;
+    for (size_t j = 0; j < ly.Wm.size() && !rc; ++j) {
+      const int outw = ly.mdims[j + 1], outp = round_up(outw, 32), ka = round_up(curw, 32);
+      rc |= gemm(cur, j == 0 ? D : ka, ly.Wm[j], ly.Wm_lo[j], (long)outw * ka, ka, MHt[l][j], outp, outw, ly.bm[j],
+                 ACT_TANH, nullptr, outp);
+      cur = MHt[l][j];
+      curw = outw;
+    }
+    if (!rc) launch_add_rows(units, D, H1t[l], D, MHt[l].back(), D, Ht[l + 1], D, s);
+  }
+  if (rc) return jq_fail(JQ_EINVAL, "Psiformer tape launch rejected");
+  return psi_outputs(elec, Ht[L], Bc, false, s);
+}
+
+// dY = dX W^T for the channel rows of every walker (walker-major row groups)
+static int gemm_nt_groups(const float* Xp, long ldx, const float* W, int K_, int ncols, float* Y, long ldy, long rows,
+                          int rgrp, long sxg, long syg, hipStream_t s) {
+  GemmArgs g{};
+  g.X = Xp;
+  g.ldx = ldx;
+  g.Wt = W;
+  g.ldw = K_;
+  g.Y = Y;
+  g.ldy = ldy;
+  g.rows = (int)rows;
+  g.K = K_;
+  g.ncols = ncols;
+  g.R = 1;
+  g.act = ACT_NONE;
+  g.res_scale = 1.f;
+  g.rgrp = rgrp;
+  g.sxg = sxg;
+  g.syg = syg;
+  return launch_gemm(g, PREC_F32, false, s);
+}
+
+int jq_model::vjp_psiformer(const float* elec, int B, const double* cot, hipStream_t s) {
+  const long KN = (long)K * N;
+  const bool pre = ln_mode == JQ_LN_PRE;
+  auto wg = [&](long rows, const float* X, long ldx, int Kx, const float* dY, long ldy, int Ny, long off) {
+    return launch_wgrad(rows, X, ldx, Kx, dY, ldy, Ny, Pw, pw_cap, gacc, off, 0, Ny, s);
+  };
+  for (int b0 = 0; b0 < B; b0 += gchunk) {
+    const int Bc = std::min(gchunk, B - b0);
+    const long units = (long)Bc * N;
+    const float* ec = elec + (long)b0 * T;
+    const FermiDims fd = dims(Bc, false);
+    int rc = forward_psiformer_tape(ec, Bc, s);
+    if (rc) return rc;
+    if (launch_det_vjp(fd, Phi, ldphi, env, det, cot, b0, dPhi, ldko, dEnv, s))
+      return jq_fail(JQ_EINVAL, "determinant adjoint supports N <= 64");
+    if (d.envelope != JQ_ENV_NULL)
+      launch_env_grad(fd, ec, atoms, pi[0], pi[1], sigma[0], sigma[1], d.envelope, norb, dEnv, gacc, off_env[0],
+                      off_env[1], s);
+    // orbital projection: weight (and bias) gradients from the channel's rows,
+    // dH_L = dPhi W_orb^T per channel
+    float* gH = dH[0];  // adjoint of the residual stream
+    for (int c = 0; c < nch; ++c) {
+      const int set = norb == 2 ? c : 0;
+      const int i0 = (nch == 1 || c == 0) ? 0 : n_up, ns = nch == 1 ? N : (c == 0 ? n_up : n_dn);
+      const float* dP = dPhi;
+      if (nch == 2) {
+        launch_mask_rows(units, N, i0, i0 + ns, (int)ldko, dPhi, ldko, dPhiM, ldko, s);
+        dP = dPhiM;
+      }
+      rc = wg(units, Ht[L], D, D, dP, ldko, (int)KN, off_orb[set]);
+      if (borb[c]) rc |= wg(units, nullptr, 0, 1, dP, ldko, (int)KN, off_borb[set]);
+      rc |= gemm_nt_groups(dPhi + (long)i0 * ldko, ldko, Worb_o[set], (int)ldko, D, gH + (long)i0 * D, D,
+                           (long)Bc * ns, ns, (long)N * ldko, (long)N * D, s);
+      if (rc) return jq_fail(JQ_EINVAL, "orbital adjoint launch rejected");
+    }
+    for (int l = L - 1; l >= 0 && !rc; --l) {
+      const PsiLayer& ly = pl[l];
+      // MLP: dm starts as the adjoint of the layer output (H' = H1 + m)
+      const int nm = (int)ly.Wm.size();
+      const float* dm = gH;
+      long lddm = D;
+      int mi = 0;
+      for (int j = nm - 1; j >= 0 && !rc; --j) {
+        const int inw = ly.mdims[j], outw = ly.mdims[j + 1], outp = round_up(outw, 32);
+        if (outp != outw) launch_fill_f32(dZ, units * outp, 0.f, s);
+        launch_tanh_bwd(units, outw, MHt[l][j], outp, nullptr, 0, dm, lddm, 0, dZ, outp, nullptr, 0, s);
+        const float* xin = j == 0 ? M0t[l] : MHt[l][j - 1];
+        const long ldxin = j == 0 ? D : round_up(inw, 32);
+        rc = wg(units, xin, ldxin, inw, dZ, outp, outw, ly.off_mw[j]);
+        rc |= wg(units, nullptr, 0, 1, dZ, outp, outw, ly.off_mb[j]);
+        const long ldn = round_up(inw, 32);
+        rc |= gemm_nt(dZ, outp, ly.Wm_o[j], outp, inw, dM[mi], ldn, units, s);
+        dm = dM[mi];
+        lddm = ldn;
+        mi ^= 1;
+      }
+      if (rc) return jq_fail(JQ_EINVAL, "MLP adjoint launch rejected");
+      // LN2 (pre-LN): dH1 = dH' + LN2'(dm); with no LayerNorm dH1 = dH' + dm
+      if (pre) {
+        if (launch_ln_bwd(units, D, H1t[l], D, ly.ln2_s, dm, lddm, 1e-5f, gH, D, dH1, D, Pln, D, s))
+          return jq_fail(JQ_EINVAL, "LayerNorm adjoint supports D <= 512");
+        rc = wg(units, nullptr, 0, 1, Pln, D, D, ly.off_ln2s);
+        rc |= wg(units, nullptr, 0, 1, dm, lddm, D, ly.off_ln2b);
+      } else {
+        launch_add_rows(units, D, gH, D, dm, lddm, dH1, D, s);
+      }
+      // attention output projection and the attention itself
+      rc |= wg(units, Ot[l], D, D, dH1, D, D, ly.off_wo);
+      rc |= wg(units, nullptr, 0, 1, dH1, D, D, ly.off_bo);
+      rc |= gemm_nt(dH1, D, ly.Wo_o, D, D, dO, D, units, s);
+      if (rc) return jq_fail(JQ_EINVAL, "attention projection adjoint launch rejected");
+      if (launch_attn_bwd(QKVt[l], dO, D, dQKV, 3L * D, Bc, N, H, dh, s))
+        return jq_fail(JQ_EINVAL, "attention adjoint supports N <= 64 and heads_dim <= 64");
+      const long offw[3] = {ly.off_wq, ly.off_wk, ly.off_wv}, offb[3] = {ly.off_bq, ly.off_bk, ly.off_bv};
+      for (int p = 0; p < 3; ++p) {
+        rc |= wg(units, XINt[l], D, D, dQKV + (long)p * D, 3L * D, D, offw[p]);
+        rc |= wg(units, nullptr, 0, 1, dQKV + (long)p * D, 3L * D, D, offb[p]);
+      }
+      rc |= gemm_nt(dQKV, 3L * D, ly.Wqkv_o, 3 * D, D, dXin, D, units, s);
+      if (rc) return jq_fail(JQ_EINVAL, "attention input adjoint launch rejected");
+      // LN1 (pre-LN): dH = dH1 + LN1'(dXin); with no LayerNorm dH = dH1 + dXin
+      if (pre) {
+        launch_ln_bwd(units, D, Ht[l], D, ly.ln1_s, dXin, D, 1e-5f, dH1, D, gH, D, Pln, D, s);
+        rc = wg(units, nullptr, 0, 1, Pln, D, D, ly.off_ln1s);
+        rc |= wg(units, nullptr, 0, 1, dXin, D, D, ly.off_ln1b);
+      } else {
+        launch_add_rows(units, D, dH1, D, dXin, D, gH, D, s);
+      }
+    }
+    if (rc) return jq_fail(JQ_EINVAL, "layer adjoint launch rejected");
+    // input projection
+    rc = wg(units, F0t, kin, 4 * M + 1, gH, D, D, off_win);
+    rc |= wg(units, nullptr, 0, 1, gH, D, D, off_bin);
+    if (d.jastrow) {
+      launch_jastrow_grad(ec, Bc, N, n_up, jas[0], jas[1], cot, b0, jtmp, s);
+      rc |= wg(Bc, nullptr, 0, 1, jtmp, 2, 2, off_jas);
+    }
+    if (rc) return jq_fail(JQ_EINVAL, "input adjoint launch rejected");
+  }
+  // the pack's q kernel and bias enter as 1/sqrt(dh) times the device's
+  const double qs = 1.0 / std::sqrt((double)dh);
+  for (const PsiLayer& ly : pl) {
+    launch_scale_f64(gacc + ly.off_wq, (long)D * D, qs, s);
+    launch_scale_f64(gacc + ly.off_bq, D, qs, s);
+  }
   return 0;
 }
 

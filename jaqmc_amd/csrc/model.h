@@ -69,6 +69,15 @@ struct PsiLayer {
   std::vector<void*> Wm, Wm_lo;              // MLP (out, in_pad) transposed
   std::vector<float*> bm;
   std::vector<int> mdims;                    // D, hidden..., D
+  // parameter-gradient pass: FP32 weights in their original (in, out)
+  // orientation (the q columns carrying 1/sqrt(dh)), output widths padded to
+  // 32 for the adjoint GEMMs, and pack offsets
+  float* Wqkv_o = nullptr;                   // (D, 3D)
+  float* Wo_o = nullptr;                     // (D, D)
+  std::vector<float*> Wm_o;                  // (in, round_up(out, 32))
+  long off_ln1s = 0, off_ln1b = 0, off_wq = 0, off_bq = 0, off_wk = 0, off_bk = 0, off_wv = 0, off_bv = 0;
+  long off_wo = 0, off_bo = 0, off_ln2s = 0, off_ln2b = 0;
+  std::vector<long> off_mw, off_mb;
 };
 
 struct jq_model {
@@ -120,6 +129,19 @@ struct jq_model {
   int alloc_grad_workspace(size_t tape_budget);
   int forward_ferminet_tape(const float* elec_chunk, int Bc, hipStream_t s);
   int vjp(const float* elec, int B, const double* cot, float* grad, hipStream_t s);
+  // Psiformer (pre-LN or no LayerNorm): offsets of the input layer, orbital
+  // biases and Jastrow; the tape H_0 .. H_L (residual stream), per layer
+  // LN1 output, head-split Q/K/V, attention output, H1 (after the attention
+  // residual), LN2 output and the MLP outputs; the features F0
+  long off_win = 0, off_bin = 0, off_borb[2] = {0, 0}, off_jas = 0;
+  float* F0t = nullptr;
+  std::vector<float*> Ht, XINt, QKVt, Ot, H1t, M0t;
+  std::vector<std::vector<float*>> MHt;
+  float *dH1 = nullptr, *dM[2] = {nullptr, nullptr}, *dO = nullptr, *dQKV = nullptr, *dXin = nullptr;
+  float *Pln = nullptr, *dPhiM = nullptr, *jtmp = nullptr;
+  int alloc_grad_workspace_psi(size_t tape_budget);
+  int forward_psiformer_tape(const float* elec_chunk, int Bc, hipStream_t s);
+  int vjp_psiformer(const float* elec, int B, const double* cot, hipStream_t s);
 
   // workspace (sized for `chunk` walkers in Laplacian mode; forward mode reuses
   // it for `chunk_fwd` walkers)
@@ -160,6 +182,8 @@ struct jq_model {
               double* e_pot, double* e_loc, float* grad, hipStream_t s, bool lp_local = false);
   int forward_ferminet(const float* elec_chunk, int Bc, bool lap, hipStream_t s);
   int forward_psiformer(const float* elec_chunk, int Bc, bool lap, hipStream_t s);
+  // orbital projection of the last residual stream x, envelope, determinants
+  int psi_outputs(const float* elec_chunk, const float* x, int Bc, bool lap, hipStream_t s);
   int orbitals_and_dets(const float* elec_chunk, const void* Xlast, int Bc, bool lap, hipStream_t s);
 
   int read_outputs(PackReader& rd, int d_last);

@@ -56,6 +56,8 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
   const float* Wi = rd.take((size_t)fin * D);
   const float* bi = rd.take(D);
   if (!rd.ok) return jq_fail(JQ_EINVAL, "parameter pack too short (Psiformer input)");
+  off_win = Wi - packed;
+  off_bin = bi - packed;
   if ((options & JQ_OPT_NO_INPUT_BIAS) && !all_zero(bi, D))
     return jq_fail(JQ_EINVAL, "input_bias=False but the pack's input bias is not zero");
   {
@@ -127,14 +129,69 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
       if (!rc) rc = upload(mb[j], 4L * b, (void**)&ly.bm[j]);
     }
     if (rc) return rc;
+    if (grads) {  // pack offsets and original-orientation FP32 weights
+      ly.off_ln1s = l1s - packed;
+      ly.off_ln1b = l1b - packed;
+      ly.off_wq = wq - packed;
+      ly.off_bq = bq - packed;
+      ly.off_wk = wk - packed;
+      ly.off_bk = bk - packed;
+      ly.off_wv = wv - packed;
+      ly.off_bv = bv - packed;
+      ly.off_wo = wo - packed;
+      ly.off_bo = bo - packed;
+      ly.off_ln2s = l2s - packed;
+      ly.off_ln2b = l2b - packed;
+      std::vector<float> wo3((size_t)D * 3 * D);
+      for (int r = 0; r < D; ++r)
+        for (int n = 0; n < D; ++n) {
+          wo3[(size_t)r * 3 * D + n] = qs * wq[(size_t)r * D + n];
+          wo3[(size_t)r * 3 * D + D + n] = wk[(size_t)r * D + n];
+          wo3[(size_t)r * 3 * D + 2 * D + n] = wv[(size_t)r * D + n];
+        }
+      rc = upload(wo3.data(), wo3.size() * 4, (void**)&ly.Wqkv_o);
+      if (!rc) rc = upload(wo, 4L * D * D, (void**)&ly.Wo_o);
+      ly.Wm_o.assign(mw.size(), nullptr);
+      ly.off_mw.resize(mw.size());
+      ly.off_mb.resize(mw.size());
+      for (size_t j = 0; j < mw.size() && !rc; ++j) {
+        const int a = ly.mdims[j], b = ly.mdims[j + 1], bp = round_up(b, 32);
+        std::vector<float> w2((size_t)a * bp, 0.f);
+        for (int r = 0; r < a; ++r)
+          for (int n = 0; n < b; ++n) w2[(size_t)r * bp + n] = mw[j][(size_t)r * b + n];
+        rc = upload(w2.data(), w2.size() * 4, (void**)&ly.Wm_o[j]);
+        ly.off_mw[j] = mw[j] - packed;
+        ly.off_mb[j] = mb[j] - packed;
+      }
+      if (rc) return rc;
+    }
   }
   // outputs (orbital projection, envelope), then the Jastrow
+  const size_t pos_out = rd.pos;
   {
     int rc = read_outputs(rd, D);
     if (rc) return rc;
   }
   const float* jas_p = rd.take(2);
   if (!jas_p) return jq_fail(JQ_EINVAL, "parameter pack too short (Jastrow)");
+  if (grads) {  // output offsets (orbital kernels, orbital biases, envelopes) and the Jastrow
+    const long KN = (long)K * N;
+    ldko = round_up((int)KN, 32);
+    const long ne = (long)N * M * K;
+    const long nb = (options & JQ_OPT_ORBITAL_BIAS) ? (long)norb * KN : 0;
+    for (int c = 0; c < norb; ++c) {
+      off_orb[c] = (long)pos_out + c * D * KN;
+      off_borb[c] = (long)pos_out + norb * D * KN + c * KN;
+      off_env[c] = (long)pos_out + norb * D * KN + nb + c * 2 * ne;
+      std::vector<float> wo((size_t)D * ldko, 0.f);
+      for (int r = 0; r < D; ++r)
+        for (long n = 0; n < KN; ++n) wo[(size_t)r * ldko + n] = packed[off_orb[c] + r * KN + n];
+      int rc = upload(wo.data(), wo.size() * 4, (void**)&Worb_o[c]);
+      if (rc) return rc;
+    }
+    if (norb == 1) Worb_o[1] = Worb_o[0];
+    off_jas = jas_p - packed;
+  }
   jas[0] = jas_p[0];
   jas[1] = jas_p[1];
   if (rd.pos != n_floats) return jq_fail(JQ_EINVAL, "parameter pack size mismatch");
@@ -180,7 +237,9 @@ int jq_model::init_psiformer(const float* packed, size_t n_floats) {
   for (int i = 0; i < 6; ++i)
     if (!P[i]) return jq_fail(JQ_ENOMEM, "workspace allocation failed");
   if (!Phi || !env || !det) return jq_fail(JQ_ENOMEM, "workspace allocation failed");
-  return alloc_mh_scratch();
+  int rc = alloc_mh_scratch();
+  if (!rc && grads) rc = alloc_grad_workspace_psi(4ull << 30);
+  return rc;
 }
 
 int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t s) {
@@ -417,6 +476,12 @@ int jq_model::forward_psiformer(const float* elec, int Bc, bool lap, hipStream_t
     }
   }
   if (rc) return jq_fail(JQ_EINVAL, "Psiformer launch rejected");
+  return psi_outputs(elec, x, Bc, lap, s);
+}
+
+int jq_model::psi_outputs(const float* elec, const float* x, int Bc, bool lap, hipStream_t s) {
+  const FermiDims fd = dims(Bc, lap);
+  const int RP = fd.Rp;
   // orbitals: per spin channel, batched over walkers (walker-major units)
   for (int c = 0; c < nch; ++c) {
     const int i0 = (nch == 1 || c == 0) ? 0 : n_up;

@@ -129,7 +129,8 @@ class DeviceModel:
 
     def logpsi_vjp(self, elec, cot):
         """sum_w cot[w] d log|psi(x_w)| / d theta as a float32 vector in pack
-        order (FermiNet models created with with_gradients=True)."""
+        order (models created with with_gradients=True; Psiformer with pre-LN
+        or no LayerNorm)."""
         B = self._check_elec(elec)
         if not (cot.is_cuda and cot.dtype == torch.float64 and cot.numel() == B and cot.is_contiguous()):
             raise ValueError("cot must be a contiguous float64 CUDA tensor of B entries")

@@ -112,8 +112,9 @@ class PsiformerWavefunction(_MoleculeWavefunction):
 
     def __init__(self, nspins, n_atoms, ndets=16, num_layers=4, num_heads=4, heads_dim=64, mlp_hidden_dims=(256,),
                  jastrow=True, envelope="abs_isotropic", precision="fp32", chunk_walkers=0, layer_norm_mode="pre",
-                 with_bias=True, input_bias=True, bias_orbitals=False, rescale=True, orbitals_spin_split=True):
-        super().__init__(precision, chunk_walkers)
+                 with_bias=True, input_bias=True, bias_orbitals=False, rescale=True, orbitals_spin_split=True,
+                 with_gradients=False):
+        super().__init__(precision, chunk_walkers, with_gradients)
         self.spec = ModelSpec("psiformer", tuple(nspins), n_atoms, ndets=ndets, num_layers=num_layers,
                               num_heads=num_heads, heads_dim=heads_dim, mlp_hidden=list(mlp_hidden_dims),
                               jastrow=jastrow, envelope=envelope, layer_norm_mode=layer_norm_mode,

@@ -185,7 +185,8 @@ def param_arrays(spec: ModelSpec, p):
     for c, (pi, sg) in enumerate(p["envelope"]):
         out += [(f"envelope{c}.pi", pi), (f"envelope{c}.sigma", sg)]
     if spec.kind == "psiformer":
-        out.append(("jastrow", np.asarray(p["jastrow"], np.float32)))
+        j = p["jastrow"]
+        out.append(("jastrow", j if hasattr(j, "shape") else np.asarray(j, np.float32)))
     return out
 
 

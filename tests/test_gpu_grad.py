@@ -1,4 +1,5 @@
-"""Parameter gradients of FermiNet log|psi| on the GPU (jq_logpsi_vjp) against
+"""Parameter gradients of FermiNet and Psiformer log|psi| on the GPU
+(jq_logpsi_vjp) against
 float64 torch autograd of the test restatement (itself pinned against the
 numpy oracle's log|psi| and its finite differences along parameter
 directions, tests/test_oracle_pins.py), and the LossAndGrad assembly
@@ -79,6 +80,90 @@ def test_logpsi_vjp_matches_torch_autograd(cuda, case):
     r2 = _torch_grad(spec, p, x, sysm.atoms, one)
     assert np.all(np.abs(g2 - r2) <= 1e-4 * np.abs(r2).max() + 1e-3 * np.abs(r2))
     dm.close()
+
+
+PSI_CASES = {
+    # pre-LN, two layers, one hidden MLP width that is not a multiple of 32
+    "psi_pre": (dict(num_layers=2, num_heads=2, heads_dim=32, mlp_hidden=[40], ndets=2), (2, 2)),
+    # no LayerNorm, shared orbitals with biases, isotropic envelope, unscaled features, MLP without hidden layer
+    "psi_null_shared": (dict(num_layers=1, num_heads=2, heads_dim=32, mlp_hidden=[], ndets=3, layer_norm_mode="null",
+                             rescale=False, bias_orbitals=True, orbitals_spin_split=False, envelope="isotropic"),
+                        (3, 1)),
+    # spin-polarised, 64-wide heads, two hidden MLP layers
+    "psi_polarized": (dict(num_layers=1, num_heads=2, heads_dim=64, mlp_hidden=[24, 48], ndets=2), (3, 0)),
+}
+
+
+def _psi_setup(case, B):
+    kw, nspins = PSI_CASES[case]
+    ne = nspins[0] + nspins[1]
+    charges = [float(ne - 1), 1.0]
+    sysm = S.make_system(case, [[0.0, 0.0, 0.0], [0.0, 0.3, 1.5]], charges, nspins)
+    spec = P.ModelSpec("psiformer", nspins, 2, **kw)
+    p = P.init_params(spec, seed=41, bias_scale=0.1, env_jitter=0.2)
+    p["jastrow"] = np.array([0.8, 1.3], np.float32)
+    x = S.init_walkers(sysm, B, seed=42)
+    return sysm, spec, p, x
+
+
+def _torch_grad_unused(spec, p, x, atoms, cot):
+    """As _torch_grad, with zeros for parameters the model leaves unused (the
+    LayerNorm slots of layer_norm_mode='null')."""
+    import torch
+
+    from torch_ref import flat_leaves, param_tree_to_torch, torch_logpsi
+
+    pt = param_tree_to_torch(P.cast_params(p, np.float64))
+    f = sum(float(c) * torch_logpsi(spec, pt, torch.from_numpy(x[w].astype(np.float64)),
+                                    torch.from_numpy(np.asarray(atoms, np.float64)))
+            for w, c in enumerate(cot))
+    leaves = flat_leaves(spec, pt)
+    g = torch.autograd.grad(f, leaves, allow_unused=True)
+    return np.concatenate([(gi if gi is not None else torch.zeros_like(li)).reshape(-1).detach().numpy()
+                           for gi, li in zip(g, leaves)])
+
+
+@pytest.mark.parametrize("case", list(PSI_CASES))
+def test_psiformer_logpsi_vjp_matches_torch_autograd(cuda, case):
+    """Psiformer: input projection, LayerNorms, Q/K/V (with the 1/sqrt(dh)
+    scale of the pack's q kernel), attention, output projection, MLP, orbital
+    kernels and biases, envelopes and the Jastrow parameters."""
+    import torch
+
+    from jaqmc_amd.device import DeviceModel
+
+    print(f"[grad] {case}", flush=True)
+    sysm, spec, p, x = _psi_setup(case, 6)
+    dm = DeviceModel(spec, sysm.atoms, sysm.charges, p, chunk_walkers=4, with_gradients=True)
+    cot = np.array([0.5, -1.25, 2.0, 0.0, 0.75, -0.3])
+    g = dm.logpsi_vjp(torch.from_numpy(x).to(cuda), torch.from_numpy(cot).to(cuda)).cpu().numpy()
+    ref = _torch_grad_unused(spec, p, x, sysm.atoms, cot)
+    assert g.shape == ref.shape == (P.pack(spec, p).size,)
+    tol = 1e-4 * np.abs(ref).max() + 1e-3 * np.abs(ref)
+    bad = np.abs(g - ref) > tol
+    names = [n for n, a in P.param_arrays(spec, p) for _ in range(a.size)]
+    worst = sorted({names[i] for i in np.flatnonzero(bad)})[:8]
+    print(f"[grad] {case}: max |d| {np.abs(g - ref).max():.2e}, |g|inf {np.abs(ref).max():.2e}, "
+          f"violations {int(bad.sum())} {worst}", flush=True)
+    assert not bad.any()
+    # every parameter group the model uses has a non-zero gradient somewhere
+    off = 0
+    for n, a in P.param_arrays(spec, p):
+        if not (spec.layer_norm_mode == "null" and ".ln" in n):
+            assert np.abs(ref[off:off + a.size]).max() > 0, n
+        off += a.size
+    dm.close()
+
+
+def test_psiformer_gradient_options_rejected():
+    """Psiformer post-LN gradients are rejected at creation, before any allocation."""
+    from jaqmc_amd import _lib
+    from jaqmc_amd.device import DeviceModel
+
+    sysm, spec, p, _ = _psi_setup("psi_pre", 2)
+    spec.layer_norm_mode = "post"
+    with pytest.raises(_lib.JqError):
+        DeviceModel(spec, sysm.atoms, sysm.charges, p, with_gradients=True)
 
 
 def test_vjp_chunking_and_zero_weights(cuda):

@@ -217,28 +217,37 @@ def test_pipeline_matches_torch_autograd(kind, kw, nspins):
         assert lp.lap[w] == pytest.approx(L, rel=1e-8, abs=1e-9)
 
 
-@pytest.mark.parametrize("kw", [dict(hidden_single=[8, 8, 8], hidden_double=[4, 4, 4], ndets=2),
-                                dict(hidden_single=[6, 6], hidden_double=[3, 3], ndets=2, envelope="isotropic",
-                                     orbitals_spin_split=False)])
-def test_torch_parameter_gradient_matches_oracle_directional_fd(kw):
+@pytest.mark.parametrize("kind,kw", [
+    ("ferminet", dict(hidden_single=[8, 8, 8], hidden_double=[4, 4, 4], ndets=2)),
+    ("ferminet", dict(hidden_single=[6, 6], hidden_double=[3, 3], ndets=2, envelope="isotropic",
+                      orbitals_spin_split=False)),
+    ("psiformer", dict(num_layers=2, num_heads=2, heads_dim=4, mlp_hidden=[5], ndets=2)),
+    ("psiformer", dict(num_layers=1, num_heads=2, heads_dim=4, mlp_hidden=[], ndets=2, layer_norm_mode="null",
+                       rescale=False, bias_orbitals=True, orbitals_spin_split=False)),
+])
+def test_torch_parameter_gradient_matches_oracle_directional_fd(kind, kw):
     """The torch evaluation's autograd parameter gradient (the reference for the
     GPU's jq_logpsi_vjp) against central finite differences of the numpy
-    oracle's log|psi| along random parameter directions."""
+    oracle's log|psi| along random parameter directions (Psiformer: every
+    layer's parameters, the 1/sqrt(dh) placement and the Jastrow alphas)."""
     from torch_ref import flat_leaves, param_tree_to_torch
 
     atoms = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.4]])
     sysm = S.make_system("pin", atoms, [2.0, 2.0], (2, 2))
-    spec = P.ModelSpec("ferminet", (2, 2), 2)
+    spec = P.ModelSpec(kind, (2, 2), 2)
     for k, v in kw.items():
         setattr(spec, k, v)
     p = P.cast_params(P.init_params(spec, seed=7, bias_scale=0.2, env_jitter=0.3), np.float64)
+    if kind == "psiformer":
+        p["jastrow"] = np.array([0.8, 1.3])
     x = S.init_walkers(sysm, 3, seed=8).astype(np.float64)
     c = np.array([0.7, -1.3, 2.1])
     pt = param_tree_to_torch(p)
     f = sum(ci * _torch_logpsi(spec, pt, torch.from_numpy(x[w]), torch.from_numpy(atoms)) for w, ci in enumerate(c))
     leaves = flat_leaves(spec, pt)
-    grads = torch.autograd.grad(f, leaves)
-    g = np.concatenate([gi.reshape(-1).numpy() for gi in grads])
+    grads = torch.autograd.grad(f, leaves, allow_unused=True)  # the LayerNorm slots of 'null' are unused
+    g = np.concatenate([(gi if gi is not None else torch.zeros_like(li)).reshape(-1).detach().numpy()
+                        for gi, li in zip(grads, leaves)])
     flat = P.pack(spec, p).astype(np.float64)
     rng = np.random.default_rng(9)
     m = spec.as_oracle()

@@ -107,7 +107,8 @@ def torch_logpsi(spec, p, x, atoms):
         iu = np.triu(np.ones((N, N)), 1)
         same = _t((spin[:, None] == spin[None, :]) * iu)
         anti = _t((spin[:, None] == 0) & (spin[None, :] == 1))
-        ap, aa = float(p["jastrow"][0]), float(p["jastrow"][1])
+        jp = _t(p["jastrow"])
+        ap, aa = jp[0], jp[1]
         lp = lp + torch.sum(same * (-0.25 * ap ** 2 / (ap + ree))) + torch.sum(anti * (-0.5 * aa ** 2 / (aa + ree)))
     return lp
 
