@@ -59,8 +59,7 @@ extern "C" {
 #define JQ_OPT_USE_LAST_LAYER (1 << 5) /* FermiNet use_last_layer=True: rejected (JQ_EINVAL)      */
 /* Library option (no reference counterpart): allocate the parameter-gradient
  * workspace at create, for jq_logpsi_vjp (FermiNet with hidden widths
- * multiples of 32; Psiformer with layer_norm_mode pre or null, N <= 64 and
- * heads_dim 32 or 64). */
+ * multiples of 32; Psiformer with N <= 64 and heads_dim 32 or 64). */
 #define JQ_OPT_WITH_GRADIENTS (1 << 16)
 
 /* Arithmetic of the layer / tangent GEMMs.  Determinants (LU, inverse,

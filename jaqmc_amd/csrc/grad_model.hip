@@ -242,10 +242,12 @@ int jq_model::vjp(const float* elec, int B, const double* cot, float* grad, hipS
 }
 
 // ---------------------------------------------------------------------------
-// Psiformer (pre-LN or no LayerNorm), value path:
-//   H_0 = F0 Win + bin;  per layer  XIN = LN1(H) (or H),  q|k|v = XIN Wqkv + b,
-//   O = MHA(q, k, v),  H1 = H + O Wo + bo,  M0 = LN2(H1) (or H1),
-//   m = tanh(... tanh(M0 W_0 + b_0) ... W_last + b_last),  H' = H1 + m;
+// Psiformer, value path:
+//   H_0 = F0 Win + bin;  per layer (pre-LN; no LayerNorm drops LN1, LN2)
+//   XIN = LN1(H),  q|k|v = XIN Wqkv + b,  O = MHA(q, k, v),  H1 = H + O Wo + bo,
+//   M0 = LN2(H1),  m = tanh(... tanh(M0 W_0 + b_0) ... W_last + b_last),
+//   H' = H1 + m;  post-LN:  U1 = H + O Wo + bo (XIN = H),  H1 = LN1(U1),
+//   M0 = H1,  U2 = H1 + m,  H' = LN2(U2);
 //   Phi = H_L W_orb (+ b_orb) per spin channel, then envelope and determinants.
 // The reverse sweep runs the same sequence backwards; each dense layer's
 // weight gradient is X^T dY over the chunk's rows (wgrad), its input adjoint
@@ -266,10 +268,10 @@ int launch_scale_f64(double* p, long n, double v, hipStream_t st);
 
 int jq_model::alloc_grad_workspace_psi(size_t budget) {
   const long KN = (long)K * N;
-  const bool pre = ln_mode == JQ_LN_PRE;
+  const bool pre = ln_mode == JQ_LN_PRE, post = ln_mode == JQ_LN_POST;
   long tape = kin + (long)(L + 1) * D, wmax = D;
   for (const PsiLayer& ly : pl) {
-    tape += (pre ? 2L : 0L) * D + 3L * D + 2L * D;
+    tape += (pre || post ? 2L : 0L) * D + 3L * D + 2L * D;
     for (size_t j = 1; j < ly.mdims.size(); ++j) {
       tape += round_up(ly.mdims[j], 32);
       wmax = std::max(wmax, (long)round_up(ly.mdims[j], 32));
@@ -287,6 +289,8 @@ int jq_model::alloc_grad_workspace_psi(size_t budget) {
   Ot.assign(L, nullptr);
   H1t.assign(L, nullptr);
   M0t.assign(L, nullptr);
+  U1t.assign(L, nullptr);
+  U2t.assign(L, nullptr);
   MHt.assign(L, {});
   bool ok = F0t != nullptr;
   for (int l = 0; l <= L; ++l) ok = ok && (Ht[l] = fa(U * D)) != nullptr;
@@ -296,7 +300,11 @@ int jq_model::alloc_grad_workspace_psi(size_t budget) {
     Ot[l] = fa(U * D);
     H1t[l] = fa(U * D);
     M0t[l] = pre ? fa(U * D) : H1t[l];
-    ok = XINt[l] && QKVt[l] && Ot[l] && H1t[l] && M0t[l];
+    if (post) {
+      U1t[l] = fa(U * D);
+      U2t[l] = fa(U * D);
+    }
+    ok = XINt[l] && QKVt[l] && Ot[l] && H1t[l] && M0t[l] && (!post || (U1t[l] && U2t[l]));
     for (size_t j = 1; j < pl[l].mdims.size() && ok; ++j) {
       MHt[l].push_back(fa(U * round_up(pl[l].mdims[j], 32)));
       ok = MHt[l].back() != nullptr;
@@ -333,7 +341,7 @@ int jq_model::alloc_grad_workspace_psi(size_t budget) {
 int jq_model::forward_psiformer_tape(const float* elec, int Bc, hipStream_t s) {
   const FermiDims fd = dims(Bc, false);
   const long units = (long)Bc * N;
-  const bool pre = ln_mode == JQ_LN_PRE;
+  const bool pre = ln_mode == JQ_LN_PRE, post = ln_mode == JQ_LN_POST;
   auto gemm = [&](const float* X, long ldx, void* Wt, void* Wlo, long wplane, int K_, float* Y, long ldy, int ncols,
                   const float* bias, int act, const float* Res, int zpad) {
     GemmArgs g{};
@@ -399,7 +407,8 @@ int jq_model::forward_psiformer_tape(const float* elec, int Bc, hipStream_t s) {
     aa.dh = dh;
     aa.nz = Bc * H;
     rc |= launch_attn_fwd(aa, s);
-    rc |= gemm(Ot[l], D, ly.Wo, ly.Wo_lo, (long)D * D, D, H1t[l], D, D, ly.bo, ACT_NONE, Ht[l], 0);
+    rc |= gemm(Ot[l], D, ly.Wo, ly.Wo_lo, (long)D * D, D, post ? U1t[l] : H1t[l], D, D, ly.bo, ACT_NONE, Ht[l], 0);
+    if (post) rc |= launch_layernorm_lap((int)units, 1, 1, D, U1t[l], H1t[l], ly.ln1_s, ly.ln1_b, 1e-5f, s);
     if (pre) rc |= launch_layernorm_lap((int)units, 1, 1, D, H1t[l], M0t[l], ly.ln2_s, ly.ln2_b, 1e-5f, s);
     const float* cur = M0t[l];
     int curw = D;
@@ -410,7 +419,8 @@ int jq_model::forward_psiformer_tape(const float* elec, int Bc, hipStream_t s) {
       cur = MHt[l][j];
       curw = outw;
     }
-    if (!rc) launch_add_rows(units, D, H1t[l], D, MHt[l].back(), D, Ht[l + 1], D, s);
+    if (!rc) launch_add_rows(units, D, H1t[l], D, MHt[l].back(), D, post ? U2t[l] : Ht[l + 1], D, s);
+    if (post) rc |= launch_layernorm_lap((int)units, 1, 1, D, U2t[l], Ht[l + 1], ly.ln2_s, ly.ln2_b, 1e-5f, s);
   }
   if (rc) return jq_fail(JQ_EINVAL, "Psiformer tape launch rejected");
   return psi_outputs(elec, Ht[L], Bc, false, s);
@@ -440,7 +450,7 @@ static int gemm_nt_groups(const float* Xp, long ldx, const float* W, int K_, int
 
 int jq_model::vjp_psiformer(const float* elec, int B, const double* cot, hipStream_t s) {
   const long KN = (long)K * N;
-  const bool pre = ln_mode == JQ_LN_PRE;
+  const bool pre = ln_mode == JQ_LN_PRE, post = ln_mode == JQ_LN_POST;
   auto wg = [&](long rows, const float* X, long ldx, int Kx, const float* dY, long ldy, int Ny, long off) {
     return launch_wgrad(rows, X, ldx, Kx, dY, ldy, Ny, Pw, pw_cap, gacc, off, 0, Ny, s);
   };
@@ -475,9 +485,15 @@ int jq_model::vjp_psiformer(const float* elec, int B, const double* cot, hipStre
     }
     for (int l = L - 1; l >= 0 && !rc; --l) {
       const PsiLayer& ly = pl[l];
-      // MLP: dm starts as the adjoint of the layer output (H' = H1 + m)
+      if (post) {  // H' = LN2(U2): the adjoint of U2 (= that of m) into dH1
+        if (launch_ln_bwd(units, D, U2t[l], D, ly.ln2_s, gH, D, 1e-5f, nullptr, 0, dH1, D, Pln, D, s))
+          return jq_fail(JQ_EINVAL, "LayerNorm adjoint supports D <= 512");
+        rc = wg(units, nullptr, 0, 1, Pln, D, D, ly.off_ln2s);
+        rc |= wg(units, nullptr, 0, 1, gH, D, D, ly.off_ln2b);
+      }
+      // MLP: dm starts as the adjoint of the MLP output (H' = H1 + m, or U2)
       const int nm = (int)ly.Wm.size();
-      const float* dm = gH;
+      const float* dm = post ? dH1 : gH;
       long lddm = D;
       int mi = 0;
       for (int j = nm - 1; j >= 0 && !rc; --j) {
@@ -495,8 +511,14 @@ int jq_model::vjp_psiformer(const float* elec, int B, const double* cot, hipStre
         mi ^= 1;
       }
       if (rc) return jq_fail(JQ_EINVAL, "MLP adjoint launch rejected");
-      // LN2 (pre-LN): dH1 = dH' + LN2'(dm); with no LayerNorm dH1 = dH' + dm
-      if (pre) {
+      // LN2 (pre-LN): dH1 = dH' + LN2'(dm); with no LayerNorm dH1 = dH' + dm;
+      // post-LN: the adjoint of H1 is dU2 + dm, then H1 = LN1(U1) gives dU1 in dH1
+      if (post) {
+        launch_add_rows(units, D, dH1, D, dm, lddm, gH, D, s);
+        launch_ln_bwd(units, D, U1t[l], D, ly.ln1_s, gH, D, 1e-5f, nullptr, 0, dH1, D, Pln, D, s);
+        rc = wg(units, nullptr, 0, 1, Pln, D, D, ly.off_ln1s);
+        rc |= wg(units, nullptr, 0, 1, gH, D, D, ly.off_ln1b);
+      } else if (pre) {
         if (launch_ln_bwd(units, D, H1t[l], D, ly.ln2_s, dm, lddm, 1e-5f, gH, D, dH1, D, Pln, D, s))
           return jq_fail(JQ_EINVAL, "LayerNorm adjoint supports D <= 512");
         rc = wg(units, nullptr, 0, 1, Pln, D, D, ly.off_ln2s);
@@ -518,7 +540,7 @@ int jq_model::vjp_psiformer(const float* elec, int B, const double* cot, hipStre
       }
       rc |= gemm_nt(dQKV, 3L * D, ly.Wqkv_o, 3 * D, D, dXin, D, units, s);
       if (rc) return jq_fail(JQ_EINVAL, "attention input adjoint launch rejected");
-      // LN1 (pre-LN): dH = dH1 + LN1'(dXin); with no LayerNorm dH = dH1 + dXin
+      // LN1 (pre-LN): dH = dH1 + LN1'(dXin); otherwise (U1 = H + ..., XIN = H) dH = dH1 + dXin
       if (pre) {
         launch_ln_bwd(units, D, Ht[l], D, ly.ln1_s, dXin, D, 1e-5f, dH1, D, gH, D, Pln, D, s);
         rc = wg(units, nullptr, 0, 1, Pln, D, D, ly.off_ln1s);

@@ -17,8 +17,8 @@ namespace {
 
 constexpr int LN_MAXV = 8;  // features per lane (D <= 512)
 
-// One wavefront per row.  out = base + dx (base may alias out); P = dy xh
-// (the rows of the scale gradient).
+// One wavefront per row.  out = base + dx (base may alias out; null: out =
+// dx); P = dy xh (the rows of the scale gradient).
 __global__ __launch_bounds__(256) void ln_bwd_kernel(long rows, int D, const float* __restrict__ x, long ldx,
                                                      const float* __restrict__ s, const float* __restrict__ dy,
                                                      long lddy, float eps, const float* base, long ldb, float* out,
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void ln_bwd_kernel(long rows, int D, const flo
   for (int q = 0; q < LN_MAXV; ++q) {
     const int c = lane + 64 * q;
     if (c < D) {
-      out[r * ldo + c] = base[r * ldb + c] + (gv[q] - mg - xv[q] * mgx) * rs;
+      out[r * ldo + c] = (base ? base[r * ldb + c] : 0.f) + (gv[q] - mg - xv[q] * mgx) * rs;
       P[r * ldp + c] = dv[q] * xv[q];
     }
   }

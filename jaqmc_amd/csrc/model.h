@@ -129,13 +129,14 @@ struct jq_model {
   int alloc_grad_workspace(size_t tape_budget);
   int forward_ferminet_tape(const float* elec_chunk, int Bc, hipStream_t s);
   int vjp(const float* elec, int B, const double* cot, float* grad, hipStream_t s);
-  // Psiformer (pre-LN or no LayerNorm): offsets of the input layer, orbital
-  // biases and Jastrow; the tape H_0 .. H_L (residual stream), per layer
-  // LN1 output, head-split Q/K/V, attention output, H1 (after the attention
-  // residual), LN2 output and the MLP outputs; the features F0
+  // Psiformer: offsets of the input layer, orbital biases and Jastrow; the
+  // tape H_0 .. H_L (residual stream), per layer the attention input XIN
+  // (LN1 output pre-LN), head-split Q/K/V, attention output, H1 (after the
+  // attention block), the MLP input M0 (LN2 output pre-LN), the MLP outputs
+  // and, post-LN, the sums U1, U2 the LayerNorms normalise; the features F0
   long off_win = 0, off_bin = 0, off_borb[2] = {0, 0}, off_jas = 0;
   float* F0t = nullptr;
-  std::vector<float*> Ht, XINt, QKVt, Ot, H1t, M0t;
+  std::vector<float*> Ht, XINt, QKVt, Ot, H1t, M0t, U1t, U2t;
   std::vector<std::vector<float*>> MHt;
   float *dH1 = nullptr, *dM[2] = {nullptr, nullptr}, *dO = nullptr, *dQKV = nullptr, *dXin = nullptr;
   float *Pln = nullptr, *dPhiM = nullptr, *jtmp = nullptr;

@@ -59,8 +59,6 @@ int jq_model::init(const jq_model_desc& desc, const float* packed, size_t n_floa
   }
   grads = (options & JQ_OPT_WITH_GRADIENTS) != 0;
   if (grads && kind == JQ_KIND_PSIFORMER) {
-    if (ln_mode == JQ_LN_POST)
-      return jq_fail(JQ_EINVAL, "Psiformer parameter gradients support layer_norm_mode pre and null");
     if (!attn_fwd_supported(desc.n_up + desc.n_dn, desc.heads_dim))
       return jq_fail(JQ_EINVAL, "Psiformer parameter gradients need N <= 64 and heads_dim 32 or 64");
   }

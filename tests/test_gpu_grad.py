@@ -91,6 +91,9 @@ PSI_CASES = {
                         (3, 1)),
     # spin-polarised, 64-wide heads, two hidden MLP layers
     "psi_polarized": (dict(num_layers=1, num_heads=2, heads_dim=64, mlp_hidden=[24, 48], ndets=2), (3, 0)),
+    # post-LN, no attention biases
+    "psi_post": (dict(num_layers=2, num_heads=2, heads_dim=32, mlp_hidden=[32], ndets=2, layer_norm_mode="post",
+                      with_bias=False), (2, 2)),
 }
 
 
@@ -156,12 +159,14 @@ def test_psiformer_logpsi_vjp_matches_torch_autograd(cuda, case):
 
 
 def test_psiformer_gradient_options_rejected():
-    """Psiformer post-LN gradients are rejected at creation, before any allocation."""
+    """Psiformer gradients outside the fused attention's range (head width 128)
+    are rejected at creation."""
     from jaqmc_amd import _lib
     from jaqmc_amd.device import DeviceModel
 
-    sysm, spec, p, _ = _psi_setup("psi_pre", 2)
-    spec.layer_norm_mode = "post"
+    sysm, spec, _, _ = _psi_setup("psi_pre", 2)
+    spec.num_heads, spec.heads_dim = 1, 128
+    p = P.init_params(spec, seed=1)
     with pytest.raises(_lib.JqError):
         DeviceModel(spec, sysm.atoms, sysm.charges, p, with_gradients=True)
 

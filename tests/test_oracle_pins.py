@@ -222,6 +222,7 @@ def test_pipeline_matches_torch_autograd(kind, kw, nspins):
     ("ferminet", dict(hidden_single=[6, 6], hidden_double=[3, 3], ndets=2, envelope="isotropic",
                       orbitals_spin_split=False)),
     ("psiformer", dict(num_layers=2, num_heads=2, heads_dim=4, mlp_hidden=[5], ndets=2)),
+    ("psiformer", dict(num_layers=2, num_heads=2, heads_dim=4, mlp_hidden=[5], ndets=2, layer_norm_mode="post")),
     ("psiformer", dict(num_layers=1, num_heads=2, heads_dim=4, mlp_hidden=[], ndets=2, layer_norm_mode="null",
                        rescale=False, bias_orbitals=True, orbitals_spin_split=False)),
 ])
