@@ -219,3 +219,32 @@ def test_loss_and_grad_pipeline(cuda):
     assert out["loss"] == pytest.approx(ev.mean(), rel=1e-12)
     # the unpacked tree has the parameter tree's shapes
     assert out["grads"]["single"][0][0].shape == p["single"][0][0].shape
+
+
+def test_loss_and_grad_pipeline_psiformer(cuda):
+    """LossAndGrad with a Psiformer wavefunction: E_L -> MAD clipping ->
+    weights -> contracted gradient, against per-walker torch gradients."""
+    import torch
+
+    from jaqmc_amd.interface import EuclideanKinetic, LossAndGrad, MoleculeData, PsiformerWavefunction
+
+    sysm, spec, p, x = _psi_setup("psi_pre", 12)
+    wf = PsiformerWavefunction(spec.nspins, 2, ndets=spec.ndets, num_layers=spec.num_layers,
+                               num_heads=spec.num_heads, heads_dim=spec.heads_dim, mlp_hidden_dims=spec.mlp_hidden,
+                               with_gradients=True)
+    assert wf.spec == spec
+    data = MoleculeData(torch.from_numpy(x).to(cuda), sysm.atoms, sysm.charges)
+    stats, _ = EuclideanKinetic(wf).evaluate_batch_walkers(p, data)
+    e = (stats["energy:kinetic"] + stats["energy:potential"])
+    lg = LossAndGrad(wf, clip_method="mad", clip_scale=5.0)
+    ws, _ = lg.evaluate_batch_walkers(p, data, {"total_energy": e})
+    out = lg.finalize_stats(lg.reduce(ws), None)
+    ev = e.cpu().numpy()
+    med = np.median(ev)
+    mad = np.median(np.abs(ev - med))
+    c = np.clip(ev, med - 5 * mad, med + 5 * mad)
+    w = 2 * (c - c.mean()) / c.size
+    ref = _torch_grad_unused(spec, p, x, sysm.atoms, w)
+    g = out["grads_flat"]
+    assert np.all(np.abs(g - ref) <= 1e-4 * np.abs(ref).max() + 1e-3 * np.abs(ref))
+    assert out["grads"]["layers"][0]["wq"].shape == p["layers"][0]["wq"].shape
