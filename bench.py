@@ -200,7 +200,7 @@ def cpu_baseline(spec, system, params, x, width):
     return x.shape[0] / t, t, threads, phys, model
 
 
-CPU_SAMPLE = {"li": 1024, "n2": 512, "ring42": 12, "h60": 4}
+CPU_SAMPLE = {"li": 131072, "n2": 512, "ring42": 12, "h60": 4}
 
 
 def main():
@@ -312,8 +312,11 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        nc = min(args.cpu_sample or CPU_SAMPLE[args.workload], B)
-        v, t, threads, phys, model = cpu_baseline(spec, system, params, x0[:nc], width)
+        nc = args.cpu_sample or CPU_SAMPLE[args.workload]
+        # a sample larger than the workload's walker count (Li) draws more
+        # walkers from the same initialisation
+        xs = x0[:nc] if nc <= B else S.init_walkers(system, nc, seed=11)
+        v, t, threads, phys, model = cpu_baseline(spec, system, params, xs, width)
         cpu = {"value": v, "unit": "walker-steps/s", "cores": threads, "kind": "port",
                "sample": f"{nc} walkers x 1 walker-step (FP32 numpy oracle: MH move + dense-Jacobian E_L), "
                          f"{t:.1f}s, BLAS threads {threads}",
