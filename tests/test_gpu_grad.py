@@ -248,3 +248,26 @@ def test_loss_and_grad_pipeline_psiformer(cuda):
     g = out["grads_flat"]
     assert np.all(np.abs(g - ref) <= 1e-4 * np.abs(ref).max() + 1e-3 * np.abs(ref))
     assert out["grads"]["layers"][0]["wq"].shape == p["layers"][0]["wq"].shape
+
+
+def test_psiformer_vjp_at_the_ring42_shape(cuda):
+    """The 42-electron ring Psiformer (4 layers, 4 heads of 64, 16
+    determinants) at its real shape, two walkers, against float64 torch
+    autograd."""
+    import torch
+
+    from jaqmc_amd.device import DeviceModel
+
+    sysm, spec, *_ = S.workload("ring42")
+    p = P.init_params(spec, seed=43, bias_scale=0.05, env_jitter=0.1)
+    x = S.init_walkers(sysm, 2, seed=44)
+    cot = np.array([0.8, -0.6])
+    dm = DeviceModel(spec, sysm.atoms, sysm.charges, p, with_gradients=True)
+    g = dm.logpsi_vjp(torch.from_numpy(x).to(cuda), torch.from_numpy(cot).to(cuda)).cpu().numpy()
+    ref = _torch_grad_unused(spec, p, x, sysm.atoms, cot)
+    tol = 1e-4 * np.abs(ref).max() + 1e-3 * np.abs(ref)
+    bad = np.abs(g - ref) > tol
+    print(f"[grad] ring42: max |d| {np.abs(g - ref).max():.2e}, |g|inf {np.abs(ref).max():.2e}, "
+          f"violations {int(bad.sum())} of {g.size}", flush=True)
+    assert not bad.any()
+    dm.close()
