@@ -209,6 +209,76 @@ __global__ __launch_bounds__(256) void pair_layer_kernel(long npairs, const floa
   }
 }
 
+// The 32 -> 32 two-electron layer on v_mfma_f32_16x16x4f32: one wavefront
+// per block of 16 rows (two pairs of eight tangent rows, or 16 value rows).
+// Lane (r = lane%16, g = lane/16) supplies A = in[row r][8g + kk] and
+// B = W[8g + kk][16 nb + r] at step kk (the k order inside an MFMA step is a
+// permutation both operands share); result element v is row 4g + v, column
+// 16 nb + r, so a pair's rows 0..3 / 4..7 sit in lane groups 2p / 2p + 1 and
+// the tanh rule's sums over the Jacobian rows are an in-thread sum plus one
+// exchange with lane r ^ 16.  Same rule and residual as pair_layer_kernel.
+template <int RP2>
+__global__ __launch_bounds__(256) void pair_layer32_mfma_kernel(long npairs, const float* __restrict__ in,
+                                                                float* __restrict__ out, const float* __restrict__ W,
+                                                                const float* __restrict__ bias, int residual) {
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const long nrows = npairs * RP2;
+  const long blk = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (blk * 16 >= nrows) return;  // whole wavefront
+  const long row0 = blk * 16;
+  float wb[2][8];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) wb[nb][kk] = W[(8 * g + kk) * 32 + 16 * nb + r];
+  const long ra = row0 + r < nrows ? row0 + r : nrows - 1;
+  const float4 x0 = *(const float4*)(in + ra * 32 + 8 * g);
+  const float4 x1 = *(const float4*)(in + ra * 32 + 8 * g + 4);
+  const float xa[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+  f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+  for (int kk = 0; kk < 8; ++kk)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[kk], wb[nb][kk], acc[nb], 0, 0, 0);
+  const float r2 = 0.70710678118654752f;
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    const int col = 16 * nb + r;
+    const float bv = bias[col];
+    float o[4];
+    if constexpr (RP2 == 8) {
+      const bool lo = (g & 1) == 0;  // rows 0..3 of the pair (value, Laplacian, first Jacobian rows)
+      if (lo) acc[nb][0] += bv;
+      float sq = 0.f;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (!lo || v >= 2) sq += acc[nb][v] * acc[nb][v];
+      sq += __shfl_xor(sq, 16);
+      const float y0 = __shfl(acc[nb][0], lane & ~16);
+      const float t = tanhf(y0);
+      const float d1 = 1.f - t * t;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = d1 * acc[nb][v];
+      if (lo) {
+        o[0] = t;
+        o[1] = d1 * acc[nb][1] - 2.f * t * d1 * sq;
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = tanhf(acc[nb][v] + bv);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long row = row0 + 4 * g + v;
+      if (row < nrows) {
+        float y = o[v];
+        if (residual) y = (y + in[row * 32 + col]) * r2;
+        out[row * 32 + col] = y;
+      }
+    }
+  }
+}
+
 // Generic fallback (any widths): thread = (pair, output feature).
 template <int RP2>
 __global__ void pair_layer_generic_kernel(long npairs, const float* __restrict__ in, int fin, float* __restrict__ out,
@@ -539,10 +609,15 @@ static bool pair_layer_fast(long npairs, const float* in, int fin, float* out, i
                        res);                                                                              \
     return true;                                                                                          \
   }
+  if (fin == 32 && fout == 32) {
+    const long nblk = (npairs * RP2 + 15) / 16;
+    hipLaunchKernelGGL((pair_layer32_mfma_kernel<RP2>), dim3((unsigned)((nblk + 3) / 4)), dim3(256), 0, s, npairs, in,
+                       out, W, b, res);
+    return true;
+  }
   JQ_PL(4, 16)
   JQ_PL(16, 16)
   JQ_PL(4, 32)
-  JQ_PL(32, 32)
 #undef JQ_PL
   return false;
 }
