@@ -209,15 +209,16 @@ __global__ __launch_bounds__(256) void pair_layer_kernel(long npairs, const floa
   }
 }
 
-// The 32 -> 32 two-electron layer on v_mfma_f32_16x16x4f32: one wavefront
-// per block of 16 rows (two pairs of eight tangent rows, or 16 value rows).
-// Lane (r = lane%16, g = lane/16) supplies A = in[row r][8g + kk] and
-// B = W[8g + kk][16 nb + r] at step kk (the k order inside an MFMA step is a
+// The FIN -> 32 two-electron layer (FIN = 4 or 32) on v_mfma_f32_16x16x4f32:
+// one wavefront per block of 16 rows (two pairs of eight tangent rows, or 16
+// value rows).  With KK = FIN / 4 steps, lane (r = lane%16, g = lane/16)
+// supplies A = in[row r][KK g + kk] and B = W[KK g + kk][16 nb + r] at step kk
+// (the k order inside an MFMA step is a
 // permutation both operands share); result element v is row 4g + v, column
 // 16 nb + r, so a pair's rows 0..3 / 4..7 sit in lane groups 2p / 2p + 1 and
 // the tanh rule's sums over the Jacobian rows are an in-thread sum plus one
 // exchange with lane r ^ 16.  Same rule and residual as pair_layer_kernel.
-template <int RP2>
+template <int RP2, int FIN>
 __global__ __launch_bounds__(256) void pair_layer32_mfma_kernel(long npairs, const float* __restrict__ in,
                                                                 float* __restrict__ out, const float* __restrict__ W,
                                                                 const float* __restrict__ bias, int residual) {
@@ -226,18 +227,26 @@ __global__ __launch_bounds__(256) void pair_layer32_mfma_kernel(long npairs, con
   const long blk = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blk * 16 >= nrows) return;  // whole wavefront
   const long row0 = blk * 16;
-  float wb[2][8];
+  static_assert(FIN == 4 || FIN == 32, "input widths 4 and 32");
+  constexpr int KK = FIN / 4;
+  float wb[2][KK];
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-    for (int kk = 0; kk < 8; ++kk) wb[nb][kk] = W[(8 * g + kk) * 32 + 16 * nb + r];
+    for (int kk = 0; kk < KK; ++kk) wb[nb][kk] = W[(KK * g + kk) * 32 + 16 * nb + r];
   const long ra = row0 + r < nrows ? row0 + r : nrows - 1;
-  const float4 x0 = *(const float4*)(in + ra * 32 + 8 * g);
-  const float4 x1 = *(const float4*)(in + ra * 32 + 8 * g + 4);
-  const float xa[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+  float xa[KK];
+  if constexpr (FIN == 32) {
+    const float4 x0 = *(const float4*)(in + ra * 32 + 8 * g);
+    const float4 x1 = *(const float4*)(in + ra * 32 + 8 * g + 4);
+    xa[0] = x0.x; xa[1] = x0.y; xa[2] = x0.z; xa[3] = x0.w;
+    xa[4] = x1.x; xa[5] = x1.y; xa[6] = x1.z; xa[7] = x1.w;
+  } else {
+    xa[0] = in[ra * 4 + g];
+  }
   f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
-  for (int kk = 0; kk < 8; ++kk)
+  for (int kk = 0; kk < KK; ++kk)
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) acc[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[kk], wb[nb][kk], acc[nb], 0, 0, 0);
   const float r2 = 0.70710678118654752f;
@@ -272,7 +281,7 @@ __global__ __launch_bounds__(256) void pair_layer32_mfma_kernel(long npairs, con
       const long row = row0 + 4 * g + v;
       if (row < nrows) {
         float y = o[v];
-        if (residual) y = (y + in[row * 32 + col]) * r2;
+        if (FIN == 32 && residual) y = (y + in[row * 32 + col]) * r2;
         out[row * 32 + col] = y;
       }
     }
@@ -609,15 +618,17 @@ static bool pair_layer_fast(long npairs, const float* in, int fin, float* out, i
                        res);                                                                              \
     return true;                                                                                          \
   }
-  if (fin == 32 && fout == 32) {
+  if ((fin == 32 || fin == 4) && fout == 32) {
     const long nblk = (npairs * RP2 + 15) / 16;
-    hipLaunchKernelGGL((pair_layer32_mfma_kernel<RP2>), dim3((unsigned)((nblk + 3) / 4)), dim3(256), 0, s, npairs, in,
-                       out, W, b, res);
+    const dim3 grid32((unsigned)((nblk + 3) / 4));
+    if (fin == 32)
+      hipLaunchKernelGGL((pair_layer32_mfma_kernel<RP2, 32>), grid32, dim3(256), 0, s, npairs, in, out, W, b, res);
+    else
+      hipLaunchKernelGGL((pair_layer32_mfma_kernel<RP2, 4>), grid32, dim3(256), 0, s, npairs, in, out, W, b, res);
     return true;
   }
   JQ_PL(4, 16)
   JQ_PL(16, 16)
-  JQ_PL(4, 32)
 #undef JQ_PL
   return false;
 }
