@@ -222,11 +222,13 @@ template <int RP2, int FIN>
 __global__ __launch_bounds__(256) void pair_layer32_mfma_kernel(long npairs, const float* __restrict__ in,
                                                                 float* __restrict__ out, const float* __restrict__ W,
                                                                 const float* __restrict__ bias, int residual) {
-  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  // per-wave 16 x 32 output tile (row stride 33), written in the MFMA layout
+  // and read back as 32-byte row segments for the stores
+  __shared__ float tile[4][16 * 33];
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4, wave = threadIdx.x >> 6;
   const long nrows = npairs * RP2;
-  const long blk = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (blk * 16 >= nrows) return;  // whole wavefront
-  const long row0 = blk * 16;
+  const long blk = (long)blockIdx.x * 4 + wave;
+  const long row0 = blk * 16;  // blocks past the end compute on clamped rows and store nothing
   static_assert(FIN == 4 || FIN == 32, "input widths 4 and 32");
   constexpr int KK = FIN / 4;
   float wb[2][KK];
@@ -277,14 +279,24 @@ __global__ __launch_bounds__(256) void pair_layer32_mfma_kernel(long npairs, con
       for (int v = 0; v < 4; ++v) o[v] = tanhf(acc[nb][v] + bv);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const long row = row0 + 4 * g + v;
-      if (row < nrows) {
-        float y = o[v];
-        if (FIN == 32 && residual) y = (y + in[row * 32 + col]) * r2;
-        out[row * 32 + col] = y;
-      }
+    for (int v = 0; v < 4; ++v) tile[wave][(4 * g + v) * 33 + col] = o[v];
+  }
+  __syncthreads();
+  // lane: row lane/4 of the block, columns 8 (lane%4) .. + 7
+  const int tr = lane >> 2, tc = 8 * (lane & 3);
+  const long row = row0 + tr;
+  if (row < nrows) {
+    float y[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) y[e] = tile[wave][tr * 33 + tc + e];
+    if (FIN == 32 && residual) {
+      const float4 a = *(const float4*)(in + row * 32 + tc), b = *(const float4*)(in + row * 32 + tc + 4);
+      const float xr[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) y[e] = (y[e] + xr[e]) * r2;
     }
+    *(float4*)(out + row * 32 + tc) = float4{y[0], y[1], y[2], y[3]};
+    *(float4*)(out + row * 32 + tc + 4) = float4{y[4], y[5], y[6], y[7]};
   }
 }
 
