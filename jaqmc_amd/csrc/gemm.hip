@@ -7,6 +7,7 @@
 #include "kernels.h"
 
 int g_gemm_variant = 0;
+int g_gemm_nw = 0;
 
 int gemm_dispatch_f32(const GemmArgs& a, hipStream_t s);
 int gemm_dispatch_bf16(const GemmArgs& a, hipStream_t s);
@@ -127,10 +128,11 @@ extern "C" int jq_bench_gemm(int prec, int R, int rows, int K, int ncols, int ac
       a.Res = X;
       a.ldres = K;
     }
-    if (R <= 1) a.lrp = variant >= 1000 ? variant / 1000 : 48;  // plain rows as lrp-row tangent units
+    if (R <= 1) a.lrp = variant % 1000000 >= 1000 ? variant % 1000000 / 1000 : 48;  // plain rows as lrp-row tangent units
     a.linear_tiles = variant % 1000 >= 100 ? 1 : 0;
-    const int old = g_gemm_variant;
+    const int old = g_gemm_variant, old_nw = g_gemm_nw;
     g_gemm_variant = variant % 100;
+    g_gemm_nw = variant >= 1000000 ? 4 : 0;  // variant + 1000000: four-wave tiles only
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
@@ -143,6 +145,7 @@ extern "C" int jq_bench_gemm(int prec, int R, int rows, int K, int ncols, int ac
     (void)hipEventElapsedTime(&ms, e0, e1);
     *ms_out = ms / reps;
     g_gemm_variant = old;
+    g_gemm_nw = old_nw;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
   }

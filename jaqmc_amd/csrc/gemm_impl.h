@@ -6,7 +6,8 @@
 // spin-mean broadcast term, the orbital projection and the Psiformer
 // projections.  Both operands are K-contiguous FP32 in HBM and staged through
 // LDS in K-chunks of 32, with the next chunk prefetched into registers while
-// the current one is multiplied (weights: see the K loop).  Each of the 4 waves owns NBW 16-column blocks
+// the current one is multiplied (weights: see the K loop).  Each wave (4, or 8 in
+// 256-column tiles of two-block waves) owns NBW 16-column blocks
 // of the tile and *all* of its rows, so the cross-row reduction sum_c J_c^2 of
 // the tanh rule stays inside one wavefront (two xor-shuffles).
 //
@@ -46,6 +47,8 @@
 
 // 0: default tile table; else UPT*10 + NBW (benchmark override)
 extern int g_gemm_variant;
+// 4: no eight-wave tiles (benchmark override); 0: default
+extern int g_gemm_nw;
 
 namespace {
 
@@ -68,7 +71,7 @@ __device__ inline void split3(float x, T* t, int ns) {
 constexpr float kF16JScale = 64.f;
 
 // MBU: 16-row blocks per unit (Laplacian mode); UPT: units per tile; NBW:
-// 16-column blocks per wave.  BM = 16*MBU*UPT, BN = 64*NBW.
+// 16-column blocks per wave.  BM = 16*MBU*UPT, BN = 16*NBW*NW (NW waves).
 // Waves per SIMD the register allocation must leave room for: three for tiles
 // of at most 12 accumulator blocks, two otherwise.
 template <int MB, int NBW>
@@ -79,11 +82,15 @@ constexpr int gemm_min_waves() {
 // LNORM: the fused-LayerNorm variant (GemmArgs::Yln set; no head-split,
 // broadcast or padding outputs), a kernel of its own so that the plain
 // variant keeps its register allocation.
-template <int MBU, int UPT, int NBW, int PREC, bool LAP, bool LNORM = false>
-__global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_kernel(const GemmArgs a) {
+// NW: waves per workgroup (4, or 8 for a 256-column tile of two-block waves,
+// whose operand tile all eight share).
+template <int MBU, int UPT, int NBW, int PREC, bool LAP, bool LNORM = false, int NW = 4>
+__global__ __launch_bounds__(64 * NW, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_kernel(const GemmArgs a) {
+  static_assert(NW == 4 || (NW == 8 && !LNORM), "four waves, or eight without the fused LayerNorm");
+  constexpr int NT = 64 * NW;  // threads
   constexpr int MB = MBU * UPT;
   constexpr int BM = 16 * MB;
-  constexpr int BN = 64 * NBW;
+  constexpr int BN = 16 * NBW * NW;
   constexpr int KC = 32;
   constexpr bool F32 = PREC == PREC_F32;
   constexpr bool F16 = PREC == PREC_FP16X3;
@@ -91,7 +98,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   constexpr int LS = F32 ? 36 : 40;  // LDS row stride (elements)
   using TL = typename std::conditional<F32, float, typename std::conditional<F16, f16, bf16>::type>::type;
   constexpr int NSL = F32 ? 1 : NS;
-  // LDS holds the activation tile of one K chunk, which all four waves read;
+  // LDS holds the activation tile of one K chunk, which all the waves read;
   // the epilogue reuses it to stage output rows.  The weights are not staged:
   // every wave owns its own columns, so each lane loads its MFMA B fragments
   // (8 consecutive k of one column) straight from global memory, where the
@@ -101,9 +108,9 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // it+1 is staged into one while chunk it is multiplied from the other, so
   // the K loop needs one barrier per chunk.  Larger tiles use one buffer and
   // two barriers.
-  constexpr bool DB = 2 * A_BYTES <= 48 * 1024;
+  constexpr bool DB = 2 * A_BYTES <= (NW == 8 ? 96 : 48) * 1024;  // eight waves: one workgroup per CU
   constexpr int NBUF = DB ? 2 : 1;
-  constexpr int STG_BYTES = 4 * 16 * (NBW * 16 + 4) * 4;
+  constexpr int STG_BYTES = NW * 16 * (NBW * 16 + 4) * 4;
   constexpr int ARENA = NBUF * A_BYTES > STG_BYTES ? NBUF * A_BYTES : STG_BYTES;
   __shared__ __attribute__((aligned(16))) char arena[ARENA];
   constexpr int PLANE = BM * LS;  // elements of one operand plane
@@ -112,7 +119,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   __shared__ float rsc[F16 ? BM : 1], rsi[F16 ? BM : 1];
 
   // register prefetch: A chunk as float4 (8 per 32-wide row); B fragments
-  constexpr int AV = (BM * 8 + 255) / 256;
+  constexpr int AV = (BM * 8 + NT - 1) / NT;
   f32x4 pa[AV];
   // B fragments, in two sets used alternately by consecutive chunks (see the
   // K loop); set 1 is used only where EARLY_B holds.
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   long xo[AV];
 #pragma unroll
   for (int v = 0; v < AV; ++v) {
-    int idx = tid + v * 256;
+    int idx = tid + v * NT;
     if (idx >= BM * 8) idx -= BM * 8;  // duplicate of an in-tile load (not staged)
     const int r = idx >> 3, q = idx & 7;
     const long gr = row0 + r < a.rows ? row0 + r : (long)a.rows - 1;
@@ -211,9 +218,9 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     TL* const Ab = Abase + buf * NSL * PLANE;
 #pragma clang loop unroll(full)
     for (int v = 0; v < AV; ++v) {
-      const int idx = tid + v * 256;
+      const int idx = tid + v * NT;
       const int r = idx >> 3, q = idx & 7;
-      if ((v + 1) * 256 > BM * 8 && idx >= BM * 8) continue;  // only a last, partial pass
+      if ((v + 1) * NT > BM * 8 && idx >= BM * 8) continue;  // only a last, partial pass
       if constexpr (F32) {
         *(f32x4*)(&Ab[r * LS + q * 4]) = pa[v];
       } else {
@@ -270,7 +277,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   float csi[NBW], csf[NBW];  // inverse and plain column scales of the lane's columns (FP16X3)
   if constexpr (F16) {
     const int per = LAP ? 16 * MBU : (a.lrp > 0 ? a.lrp : 1);
-    for (int r = tid; r < BM; r += 256) {
+    for (int r = tid; r < BM; r += NT) {
       const bool in = row0 + r < a.rows;
       rsc[r] = in ? kF16JScale : 1.f;
       rsi[r] = in ? 1.f / kF16JScale : 1.f;
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
     const int off = (int)(row0 % per);
     const int cpr = per > 1 ? 2 : 1;  // scanned rows per period
     const int ncand = cpr * ((BM + off + per - 1) / per);
-    for (int q = wave; q < ncand; q += 4) {
+    for (int q = wave; q < ncand; q += NW) {
       const int r = (q / cpr) * per + (q % cpr) - off;
       const long gr = row0 + r;
       if (r < 0 || r >= BM || gr >= a.rows) continue;
@@ -600,7 +607,7 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   // unrolled, so the store code exists once; the staging write selects the
   // block's accumulators with compile-time indices.
   constexpr int WC = NBW * 16, SW = WC + 4, C4W = WC / 4, RPI = 64 / C4W;
-  static_assert(4 * 16 * SW * 4 <= ARENA, "staging slices must fit the arena");
+  static_assert(NW * 16 * SW * 4 <= ARENA, "staging slices must fit the arena");
   static_assert(64 % C4W == 0 && 16 % RPI == 0, "store pattern");
   __syncthreads();  // every wave is done with the operand tiles
   float* stg = (float*)arena + wave * 16 * SW;
@@ -893,13 +900,13 @@ __global__ __launch_bounds__(256, (gemm_min_waves<MBU * UPT, NBW>())) void gemm_
   }
 }
 
-template <int MBU, int UPT, int NBW, int PREC, bool LAP, bool LNORM = false>
+template <int MBU, int UPT, int NBW, int PREC, bool LAP, bool LNORM = false, int NW = 4>
 void launch_one(const GemmArgs& a, hipStream_t s) {
-  constexpr int BM = 16 * MBU * UPT, BN = 64 * NBW;
+  constexpr int BM = 16 * MBU * UPT, BN = 16 * NBW * NW;
   const int ntiles = a.linear_tiles ? ceil_div(a.rows, BM) * ceil_div(a.ncols, BN)
                                     : xcd_grid(ceil_div(a.rows, BM), ceil_div(a.ncols, BN));
   dim3 grid(ntiles, 1, a.nbatch > 1 ? a.nbatch : 1);
-  hipLaunchKernelGGL((gemm_kernel<MBU, UPT, NBW, PREC, LAP, LNORM>), grid, dim3(256), 0, s, a);
+  hipLaunchKernelGGL((gemm_kernel<MBU, UPT, NBW, PREC, LAP, LNORM, NW>), grid, dim3(64 * NW), 0, s, a);
 }
 
 
@@ -923,6 +930,16 @@ bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
     }
   }
   if (a.Yln) return false;
+  // FP16X3 192-row tiles of two-block waves wider than 128 columns: eight
+  // waves share one operand tile over 256 columns (the two 128-column blocks
+  // of four waves each streamed the activation panel separately)
+  if constexpr (PREC == PREC_FP16X3 && (MBU == 3 || MBU == 12)) {
+    constexpr int U8 = 12 / MBU;
+    if (upt == U8 && nbw == 2 && a.ncols > 128 && g_gemm_nw != 4) {
+      launch_one<MBU, U8, 2, PREC, true, false, 8>(a, s);
+      return true;
+    }
+  }
   JQ_TRY(1, 2)
   JQ_TRY(1, 4)
   JQ_TRY(2, 2)
