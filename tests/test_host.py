@@ -162,3 +162,35 @@ def test_equal_chunks_from_workspace_probe():
     assert equal_chunks(ws1, ws2, 10, 0) == 1                         # budget below one walker
     c = equal_chunks(ws1, ws2, 1000, fixed + 333 * per)
     assert fixed + c * per <= fixed + 333 * per and -(-1000 // c) == 4
+
+
+def test_traffic_summary_calibration_and_exclusions(tmp_path):
+    """jaqmc_amd.profiling.traffic: counters scaled by the copy calibration,
+    excluded dispatches (by grid size or ordinal) listed but not averaged."""
+    import sqlite3
+
+    from jaqmc_amd.profiling import traffic
+
+    def db(path, counter, rows):
+        con = sqlite3.connect(path)
+        con.execute("create table counters_collection (dispatch_id int, kernel_name text, grid_size int, "
+                    "counter_name text, value real)")
+        con.executemany("insert into counters_collection values (?, ?, ?, ?, ?)",
+                        [(i, n, g, counter, v) for i, (n, g, v) in enumerate(rows)])
+        con.commit()
+        con.close()
+
+    copy = 1 << 30
+    k = "void gemm_kernel<8, 1, 4, 4, true, false>(GemmArgs)"
+    # the copy reads as half its bytes on FETCH_SIZE (KiB), in full on WRITE_SIZE
+    db(tmp_path / "f.db", "FETCH_SIZE", [("__amd_rocclr_copyBuffer", 1, copy / 2048), (k, 100, 1000.0),
+                                         (k, 100, 3000.0), (k, 50, 99999.0)])
+    db(tmp_path / "w.db", "WRITE_SIZE", [("__amd_rocclr_copyBuffer", 1, copy / 1024), (k, 100, 10.0),
+                                         (k, 100, 30.0), (k, 50, 77777.0)])
+    t = traffic(str(tmp_path / "f.db"), str(tmp_path / "w.db"), "gemm_kernel<8, 1, 4, 4, true", copy, ("g50",))
+    assert t["FETCH_SIZE"]["scale"] == 2.0 and t["WRITE_SIZE"]["scale"] == 1.0
+    assert t["FETCH_SIZE"]["bytes_per_dispatch"] == 2000.0 * 1024 * 2
+    assert t["WRITE_SIZE"]["bytes_per_dispatch"] == 20.0 * 1024
+    assert len(t["FETCH_SIZE"]["each"]) == 3  # the excluded dispatch is still listed
+    t2 = traffic(str(tmp_path / "f.db"), str(tmp_path / "w.db"), "gemm_kernel<8, 1, 4, 4, true", copy, ("d0", "d2"))
+    assert t2["FETCH_SIZE"]["bytes_per_dispatch"] == 3000.0 * 1024 * 2
