@@ -933,7 +933,7 @@ bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
   // FP16X3 192-row tiles of two-block waves wider than 128 columns: eight
   // waves share one operand tile over 256 columns (the two 128-column blocks
   // of four waves each streamed the activation panel separately)
-  if constexpr (PREC == PREC_FP16X3 && (MBU == 3 || MBU == 12)) {
+  if constexpr (PREC == PREC_FP16X3 && (MBU == 3 || MBU == 4 || MBU == 6 || MBU == 12)) {
     constexpr int U8 = 12 / MBU;
     if (upt == U8 && nbw == 2 && a.ncols > 128 && g_gemm_nw != 4) {
       launch_one<MBU, U8, 2, PREC, true, false, 8>(a, s);
@@ -955,14 +955,18 @@ bool launch_lap(const GemmArgs& a, int upt, int nbw, hipStream_t s) {
 // the 16-bit modes take four units per tile (the weight fragments of a chunk
 // are reused over 12 row blocks), FP32 two.  FP16X3 at 128-row units takes
 // 256-column tiles when the layer is wider than 128 columns (the operand tile
-// is staged and split once for twice the columns).
+// is staged and split once for twice the columns).  FP16X3 layers wider than
+// 128 columns at 48-, 64-, 96- and 192-row units run 192-row tiles of eight
+// waves (R = 64: 2.41 -> 2.33 ms, R = 96: 2.78 -> 2.31 ms at K = 320, 256
+// columns).
 void default_cfg(int R, int prec, bool wide, int* upt, int* nbw) {
   *nbw = 2;
   switch (R) {
     case 16: *upt = 4; break;
     case 32: *upt = 2; break;
     case 48: *upt = prec == PREC_F32 ? 2 : 4; break;
-    case 64: *upt = 2; break;
+    case 64: *upt = prec == PREC_FP16X3 && wide ? 3 : 2; break;  // FP16X3: 192-row eight-wave tiles
+    case 96: *upt = prec == PREC_FP16X3 && wide ? 2 : 1; break;
     case 128:
       *upt = 1;
       if (prec == PREC_FP16X3 && wide) *nbw = 4;

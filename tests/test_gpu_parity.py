@@ -67,10 +67,22 @@ def _h24():
     return S.h_chain(24)
 
 
+def _h20():
+    return S.h_chain(20)
+
+
+def _h26():
+    return S.h_chain(26)
+
+
 EDGE_CASES = {
     # 17 and 24 electrons: the FP64-MFMA determinant path at padded size 32
     "h17": (_h17, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=2)),
     "h24": (_h24, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=2)),
+    # 20 and 26 electrons, 256-wide layers: the eight-wave FP16X3 layer GEMM
+    # tiles at 64- and 96-row tangent units
+    "h20_wide": (_h20, dict(hidden_single=[256, 256], hidden_double=[16, 16], ndets=2)),
+    "h26_wide": (_h26, dict(hidden_single=[256, 256], hidden_double=[16, 16], ndets=2)),
     # the reference's hydrogen demo shape: one electron, one spin channel
     "h_atom": (_hydrogen, dict(hidden_single=[32, 32], hidden_double=[8, 8], ndets=2)),
     # fully polarized: a single non-empty spin channel (split_nonempty_channels)
@@ -320,7 +332,7 @@ def test_mixed_precision_modes(cuda, prec, lp_tol, med_tol):
         assert np.all(rel <= bnd), (rel, bnd)
 
 
-@pytest.mark.parametrize("case,B", [("ring42", 8), ("h60", 8)])
+@pytest.mark.parametrize("case,B", [("ring42", 8), ("h60", 8), ("h20_wide", 4), ("h26_wide", 4)])
 @pytest.mark.parametrize("prec", ["bf16x6", "fp16x3"])
 def test_mixed_precision_large_configs(cuda, case, B, prec):
     """The mixed paths of the 42-electron ring (Psiformer) and H60 (FermiNet,
