@@ -218,6 +218,9 @@ def main():
     ap.add_argument("--serving-jobs", type=int, default=12, help="jobs in the serving sample (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=5.0, help="Poisson arrival rate of the serving sample")
     ap.add_argument("--serving-steps", type=int, default=20, help="evaluation iterations per serving job")
+    ap.add_argument("--vjp-walkers", type=int, default=0,
+                    help="time the parameter-gradient pass (jq_logpsi_vjp, FP32) on this many walkers of the "
+                         "workload instead of the evaluation step; prints its own JSON line")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="counter-calibration run for rocprofv3 --pmc: one 1 GiB device copy, then one local-energy "
                          "pass of the workload in the mixed precision; prints nothing on stdout")
@@ -248,6 +251,29 @@ def main():
         chunk = chunk_for_budget(spec, system.atoms, system.charges, params, args.mixed, B,
                                  args.workspace_gb * (1 << 30))
     log(f"walkers per workspace chunk: {chunk or 'library default'}")
+    if args.vjp_walkers > 0:
+        from jaqmc_amd.device import DeviceModel
+
+        nv = args.vjp_walkers
+        xv = torch.from_numpy(S.init_walkers(system, nv, seed=3)).cuda()
+        cot = torch.linspace(-1.0, 1.0, nv, dtype=torch.float64, device="cuda")
+        dm = DeviceModel(spec, system.atoms, system.charges, params, with_gradients=True)
+        for _ in range(args.warmup):
+            g = dm.logpsi_vjp(xv, cot)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g = dm.logpsi_vjp(xv, cot)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        ok = bool(torch.isfinite(g).all())
+        dm.close()
+        print(json.dumps({"metric": "parameter_gradient_walkers_per_s", "value": nv / dt, "unit": "walkers/s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+                          "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+                          "config": {"workload": args.workload, "walkers": nv, "parameters": int(g.numel())},
+                          "finite": ok}))
+        return
     if args.traffic_probe:
         from jaqmc_amd.device import DeviceModel
 
