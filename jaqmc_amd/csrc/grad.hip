@@ -125,7 +125,9 @@ __global__ __launch_bounds__(256) void det_vjp_kernel(FermiDims d, const float* 
   }
 }
 
-// Envelope parameter gradients, one thread per (orbital set, o, I, k):
+// Envelope parameter gradients, one wavefront per (orbital set, o, I, k),
+// its lanes splitting the (walker, electron) terms and summing in FP64 in a
+// fixed order:
 //   d pi[o,I,k]    += sum_{w, i in set} denv[w][i][k N + o] e_iI
 //   d sigma[o,I,k] += sum denv pi e_iI (-r_iI) (sign(sigma) for abs_isotropic)
 // accumulated in FP64 into acc (pack layout (N, M, K) per set: pi then sigma,
@@ -137,7 +139,9 @@ __global__ void env_grad_kernel(FermiDims d, const float* __restrict__ elec, con
   const int N = d.N, M = d.M, K = d.K, KN = K * N;
   const long per = (long)N * M * K;
   const long tot = per * nsets;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < tot; t += (long)gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x & 63;
+  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+  for (long t = blockIdx.x * (long)(blockDim.x >> 6) + (threadIdx.x >> 6); t < tot; t += nwaves) {
     const int set = (int)(t / per);
     const long q = t % per;  // (o, I, k) in pack order
     const int o = (int)(q / ((long)M * K)), I = (int)((q / K) % M), k = (int)(q % K);
@@ -152,19 +156,24 @@ __global__ void env_grad_kernel(FermiDims d, const float* __restrict__ elec, con
       i1 = set == 0 ? d.n_up : N;
     }
     double gp = 0.0, gs = 0.0;
-    for (int w = 0; w < d.Bc; ++w)
-      for (int i = i0; i < i1; ++i) {
-        const float* x = elec + ((long)w * N + i) * 3;
-        const float dx = x[0] - atoms[3 * I], dy = x[1] - atoms[3 * I + 1], dz = x[2] - atoms[3 * I + 2];
-        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-        const float e = expf(-a * r);
-        const double de = denv[((long)w * N + i) * KN + k * N + o];
-        gp += de * e;
-        gs += de * (double)(pi * e * (-r) * sgn);
-      }
-    const long off = set ? off1 : off0;
-    acc[off + q] += gp;
-    acc[off + per + q] += gs;
+    const int ni = i1 - i0;
+    for (long wi = lane; wi < (long)d.Bc * ni; wi += 64) {
+      const int w = (int)(wi / ni), i = i0 + (int)(wi % ni);
+      const float* x = elec + ((long)w * N + i) * 3;
+      const float dx = x[0] - atoms[3 * I], dy = x[1] - atoms[3 * I + 1], dz = x[2] - atoms[3 * I + 2];
+      const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float e = expf(-a * r);
+      const double de = denv[((long)w * N + i) * KN + k * N + o];
+      gp += de * e;
+      gs += de * (double)(pi * e * (-r) * sgn);
+    }
+    gp = wave_sum_d(gp);
+    gs = wave_sum_d(gs);
+    if (lane == 0) {
+      const long off = set ? off1 : off0;
+      acc[off + q] += gp;
+      acc[off + per + q] += gs;
+    }
   }
 }
 
@@ -351,9 +360,9 @@ int launch_det_vjp(const FermiDims& d, const float* Phi, long ldphi, const float
 int launch_env_grad(const FermiDims& d, const float* elec, const float* atoms, const float* pi0, const float* pi1,
                     const float* sg0, const float* sg1, int kind, int nsets, const float* denv, double* acc,
                     long off0, long off1, hipStream_t s) {
-  const long tot = (long)d.N * d.M * d.K * nsets;
-  hipLaunchKernelGGL(env_grad_kernel, dim3(grid_of(tot)), dim3(256), 0, s, d, elec, atoms, pi0, pi1, sg0, sg1, kind,
-                     nsets, denv, acc, off0, off1);
+  const long tot = (long)d.N * d.M * d.K * nsets;  // one wavefront each
+  hipLaunchKernelGGL(env_grad_kernel, dim3(grid_of(tot * 64)), dim3(256), 0, s, d, elec, atoms, pi0, pi1, sg0, sg1,
+                     kind, nsets, denv, acc, off0, off1);
   return 0;
 }
 
